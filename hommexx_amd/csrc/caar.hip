@@ -1,0 +1,196 @@
+// Dynamics right-hand side and update for one RK stage (CAAR), plus the
+// small element-wise kernels of the step (dp3d from ps, prim_step init,
+// the (5u1-u0)/4 stage combination).
+//
+// CAAR maps one lane to one GLL column: a wave holds 4 elements x 16 points
+// and walks the column top to bottom, so the pressure, geopotential and
+// omega scans are plain serial recurrences in registers (same order as
+// CaarFunctorImpl.hpp:621-650, 689-729, 854-889), while the horizontal
+// derivatives at each level are cross-lane operations inside the 16-lane
+// group of the element.
+//   pass 1 (top->bottom): p, T_v, div(v dp), vn0 += w v dp, grad p, omega/p,
+//                         T(np1), dp3d(np1), and Rgas*T_v*dp/(2p) into phi
+//   pass 2 (bottom->top): phi = phis + 2*sum_below + own term
+//   pass 3 (top->bottom): energy gradient, vorticity, v(np1)
+// Outputs go to pre-exchange buffers; the DSS kernel writes the state.
+#include "kernels.hpp"
+
+namespace hx {
+
+__global__ __launch_bounds__(64) void caar_kernel(CaarArgs a) {
+  const int lane = threadIdx.x;
+  const int pt = lane & 15;
+  const int ie = blockIdx.x * 4 + (lane >> 4);
+  if (ie >= a.nelem) return;
+  const int L = a.nlev;
+  ElemLane e;
+  load_elem(a.m, ie, pt, e);
+  DvvLane d;
+  load_dvv(a.m.dvv, pt, d);
+  const size_t sb = (size_t)ie * L * 16 + pt;  // scalar [ie][lev][pt]
+  const size_t vb = (size_t)ie * L * 32 + pt;  // vector [ie][lev][c][pt]
+  const double eta = a.eta_ave_w, dt = a.dt;
+  const double *qv = a.qdp_n0 ? a.qdp_n0 + (size_t)ie * a.qsize * L * 16 + pt : nullptr;
+
+  // ---------------------------------------------------------------- pass 1
+  double p_prev = a.p_top, dp_prev = 0.0, integ = 0.0;
+  for (int k = 0; k < L; ++k) {
+    const size_t s = sb + (size_t)k * 16, v = vb + (size_t)k * 32;
+    const double dp = a.dp_n0[s];
+    const double u = a.v_n0[v], w = a.v_n0[v + 16];
+    const double t = a.t_n0[s];
+    double tv = t;
+    if (qv) {
+      double qt = qv[(size_t)k * 16] / dp;
+      qt *= (RWATER_VAPOR / RGAS - 1.0);
+      qt += 1.0;
+      tv = t * qt;
+    }
+    const double p = p_prev + 0.5 * (dp_prev + dp);
+    p_prev = p;
+    dp_prev = dp;
+    const double vdp0 = u * dp, vdp1 = w * dp;
+    a.vn0[v] += eta * vdp0;
+    a.vn0[v + 16] += eta * vdp1;
+    const double divv = sph_div(e, d, vdp0, vdp1);
+    double gp0, gp1;
+    sph_grad(e, d, p, gp0, gp1);
+    const double vgrad_p = u * gp0 + w * gp1;
+    const double om = (vgrad_p - (integ + 0.5 * divv)) / p;
+    integ = integ + divv;
+    a.omega[s] += eta * om;
+    double gt0, gt1;
+    sph_grad(e, d, t, gt0, gt1);
+    const double vgrad_t = u * gt0 + w * gt1;
+    const double ttens = 0 - vgrad_t + KAPPA * tv * om;
+    double temp = ttens * dt + a.t_nm1[s];
+    temp *= e.sph;
+    a.out_t[s] = temp;
+    double tmp = 0;
+    tmp += divv;
+    tmp -= 0;
+    tmp = a.dp_nm1[s] - tmp * dt;
+    a.out_dp[s] = e.sph * tmp;
+    a.phi[s] = RGAS * tv * (dp * 0.5 / p);
+  }
+  // ---------------------------------------------------------------- pass 2
+  double integration = 0.0;
+  for (int k = L - 1; k >= 0; --k) {
+    const size_t s = sb + (size_t)k * 16;
+    const double r = a.phi[s];
+    a.phi[s] = e.phis + 2.0 * integration + r;
+    integration = integration + r;
+  }
+  // ---------------------------------------------------------------- pass 3
+  p_prev = a.p_top;
+  dp_prev = 0.0;
+  for (int k = 0; k < L; ++k) {
+    const size_t s = sb + (size_t)k * 16, v = vb + (size_t)k * 32;
+    const double dp = a.dp_n0[s];
+    const double u = a.v_n0[v], w = a.v_n0[v + 16];
+    const double t = a.t_n0[s];
+    double tv = t;
+    if (qv) {
+      double qt = qv[(size_t)k * 16] / dp;
+      qt *= (RWATER_VAPOR / RGAS - 1.0);
+      qt += 1.0;
+      tv = t * qt;
+    }
+    const double p = p_prev + 0.5 * (dp_prev + dp);
+    p_prev = p;
+    dp_prev = dp;
+    double gp0, gp1;
+    sph_grad(e, d, p, gp0, gp1);
+    double eg0 = RGAS * (tv / p) * gp0;
+    double eg1 = RGAS * (tv / p) * gp1;
+    const double ephi = 0.5 * (u * u + w * w) + a.phi[s];
+    sph_grad_update(e, d, ephi, eg0, eg1);
+    double vort = sph_vort(e, d, u, w);
+    vort += e.fcor;
+    eg0 *= -1;
+    eg0 += 0 + w * vort;
+    eg1 *= -1;
+    eg1 += 0 - u * vort;
+    eg0 *= dt;
+    eg0 += a.v_nm1[v];
+    eg1 *= dt;
+    eg1 += a.v_nm1[v + 16];
+    a.out_v[v] = e.sph * eg0;
+    a.out_v[v + 16] = e.sph * eg1;
+  }
+}
+
+void launch_caar(hipStream_t st, const CaarArgs &a) {
+  const int blocks = (a.nelem + 3) / 4;
+  hipLaunchKernelGGL(caar_kernel, dim3(blocks), dim3(64), 0, st, a);
+}
+
+// ------------------------------------------------------------------ small
+__global__ void dp3d_from_ps_kernel(int nelem, int nlev, const double *__restrict__ hyai,
+                                    const double *__restrict__ hybi, double ps0, const double *__restrict__ ps,
+                                    double *__restrict__ dp) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n = (size_t)nelem * nlev * 16;
+  if (i >= n) return;
+  const int pt = i & 15;
+  const int k = (i >> 4) % nlev;
+  const int ie = (int)((i >> 4) / nlev);
+  const double da = hyai[k + 1] - hyai[k], db = hybi[k + 1] - hybi[k];
+  dp[i] = da * ps0 + db * ps[(size_t)ie * 16 + pt];
+}
+
+void launch_dp3d_from_ps(hipStream_t st, int nelem, int nlev, const double *hyai, const double *hybi, double ps0,
+                         const double *ps, double *dp) {
+  const size_t n = (size_t)nelem * nlev * 16;
+  hipLaunchKernelGGL(dp3d_from_ps_kernel, dim3((n + 255) / 256), dim3(256), 0, st, nelem, nlev, hyai, hybi, ps0, ps,
+                     dp);
+}
+
+__global__ void prim_step_init_kernel(size_t n, bool zero_diss, const double *__restrict__ dp_n0,
+                                      double *__restrict__ eta_dot, double *__restrict__ vn0,
+                                      double *__restrict__ omega, double *__restrict__ dpa, double *__restrict__ dpb,
+                                      double *__restrict__ derived_dp) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  eta_dot[i] = 0;
+  omega[i] = 0;
+  const size_t row = i >> 4, pt = i & 15;
+  vn0[row * 32 + pt] = 0;
+  vn0[row * 32 + 16 + pt] = 0;
+  if (zero_diss) {
+    dpa[i] = 0;
+    dpb[i] = 0;
+  }
+  derived_dp[i] = dp_n0[i];
+}
+
+void launch_prim_step_init(hipStream_t st, int nelem, int nlev, bool zero_diss, const double *dp_n0,
+                           double *eta_dot, double *vn0, double *omega, double *dpa, double *dpb,
+                           double *derived_dp) {
+  const size_t n = (size_t)nelem * nlev * 16;
+  hipLaunchKernelGGL(prim_step_init_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, zero_diss, dp_n0, eta_dot,
+                     vn0, omega, dpa, dpb, derived_dp);
+}
+
+// nm1 <- (5*nm1 - n0)/4 for T, v, dp3d (prim_advance_exp.cpp:139-156)
+__global__ void rk_combo_kernel(size_t n, double *__restrict__ v_nm1, double *__restrict__ t_nm1,
+                                double *__restrict__ dp_nm1, const double *__restrict__ v_n0,
+                                const double *__restrict__ t_n0, const double *__restrict__ dp_n0) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  t_nm1[i] = (5.0 * t_nm1[i] - t_n0[i]) / 4.0;
+  const size_t row = i >> 4, pt = i & 15;
+  const size_t v0 = row * 32 + pt, v1 = v0 + 16;
+  v_nm1[v0] = (5.0 * v_nm1[v0] - v_n0[v0]) / 4.0;
+  v_nm1[v1] = (5.0 * v_nm1[v1] - v_n0[v1]) / 4.0;
+  dp_nm1[i] = (5.0 * dp_nm1[i] - dp_n0[i]) / 4.0;
+}
+
+void launch_rk_combo(hipStream_t st, int nelem, int nlev, double *v_nm1, double *t_nm1, double *dp_nm1,
+                     const double *v_n0, const double *t_n0, const double *dp_n0) {
+  const size_t n = (size_t)nelem * nlev * 16;
+  hipLaunchKernelGGL(rk_combo_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, v_nm1, t_nm1, dp_nm1, v_n0, t_n0,
+                     dp_n0);
+}
+
+}  // namespace hx

@@ -1,0 +1,1072 @@
+// Host side of libhommexx_amd.so: the reference's iso_c_binding C-ABI
+// (cxx_f90_interface.cpp, mpi/mpi_cxx_f90_interface.cpp, Hommexx_Session.cpp,
+// prim_driver.cpp) over a device-resident context, plus hommexx_hip_*
+// extensions for the test/benchmark harness.  Fortran passes scalars by
+// reference and arrays as pointers to the flat elem_* arrays; time-level
+// indices enter 1-based and leave prim_run_subcycle_c 0-based, exactly as in
+// the reference.  Errors that the reference turns into MPI_Abort abort the
+// process too unless hommexx_hip_set_abort_on_error(0) was called, in which
+// case they are recorded for hommexx_hip_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "../../include/hommexx_amd.h"
+
+using namespace hx;
+
+namespace {
+
+struct Params {
+  int remap_alg = 1, limiter_option = 8, rsplit = 1, qsplit = 1, time_step_type = 5, energy_fixer = 0;
+  int qsize = 0, state_frequency = 1, hypervis_order = 2, hypervis_subcycle = 1, ftype = -1;
+  double nu = 0, nu_p = 0, nu_q = 0, nu_s = 0, nu_div = 0, nu_top = 0, hypervis_scaling = 0;
+  double nu_ratio1 = 1, nu_ratio2 = 1;
+  bool prescribed_wind = false, moist = false, disable_diagnostics = true, use_cpstar = false, use_sl = false;
+  bool set = false;
+};
+
+struct Timer {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  int count = 0;
+  double total_ms = 0;
+};
+
+struct Ctx {
+  // configuration
+  bool session = false;
+  int nlev_cfg = 72, qsize_d_cfg = -1;
+  bool abort_on_error = true;
+  int last_error = 0;
+  std::string msg;
+  hipStream_t st = nullptr;
+  // connectivity (host)
+  int conn_nelem = -1;
+  bool conn_final = false;
+  std::vector<int> conn_rl, conn_rp;  // [ie*8 + pos] -> remote lid / pos (reference numbering)
+  // params / dims
+  Params p;
+  double dvv[16] = {0};
+  bool dvv_set = false;
+  int nelem = 0, nlev = 0, qsize = 0, qsize_d = 0;
+  bool elems_set = false, states_set = false, hv_set = false, be_set = false;
+  // 2-D metric (device)
+  double *D = nullptr, *Dinv = nullptr, *metinv = nullptr, *tensorvisc = nullptr, *vec_sph2cart = nullptr;
+  double *fcor = nullptr, *mp = nullptr, *spheremp = nullptr, *rspheremp = nullptr, *metdet = nullptr,
+         *phis = nullptr, *dvv_d = nullptr;
+  bool consthv = true;
+  int *dss_src = nullptr, *nbr = nullptr;
+  // vertical coordinate
+  double ps0 = 0, hyai0 = 0;
+  std::vector<double> hyai, hybi;
+  double *hyai_d = nullptr, *hybi_d = nullptr, *dp0_d = nullptr;
+  // states
+  double *v[3] = {}, *t[3] = {}, *dp[3] = {}, *ps[3] = {}, *qdp[2] = {}, *Q = nullptr;
+  std::vector<double> qdp_tail;  // Qdp of tracers qsize..qsize_d-1 as received (never evolved)
+  // derived
+  double *phi = nullptr, *omega = nullptr, *eta_dot = nullptr, *dpdiss_ave = nullptr, *dpdiss_bih = nullptr,
+         *derived_dp = nullptr, *divdp = nullptr, *divdp_proj = nullptr, *vn0 = nullptr;
+  // forcing
+  double *ft = nullptr, *fm = nullptr, *fq = nullptr;
+  // scratch
+  double *cv = nullptr, *ct = nullptr, *cdp = nullptr;
+  double *vt_a = nullptr, *tt_a = nullptr, *dpt_a = nullptr, *vt_b = nullptr, *tt_b = nullptr, *dpt_b = nullptr;
+  double *qdp_pre = nullptr, *qbih = nullptr, *qbih_pre = nullptr, *qlim[2] = {}, *fdss_pre = nullptr;
+  int *err_d = nullptr;
+  int *err_h = nullptr;
+  // time levels (0-based)
+  int nm1 = 0, n0 = 1, np1 = 2, nstep = 0, nstep0 = 0, n0_qdp = 0, np1_qdp = 1;
+  // debugging: stop prim_run_subcycle_c after a phase (0: never)
+  int debug_stop = 0;
+  bool stopped = false;
+  // profiling
+  bool prof = false;
+  std::map<std::string, Timer> timers;
+  std::vector<void *> allocs;
+};
+
+Ctx *g = nullptr;
+
+Ctx &C() {
+  if (!g) g = new Ctx();
+  return *g;
+}
+
+void fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  Ctx &c = C();
+  c.last_error = code;
+  c.msg = buf;
+  if (c.abort_on_error) {
+    fprintf(stderr, "hommexx_amd error %d: %s\n", code, buf);
+    fflush(stderr);
+    abort();
+  }
+}
+
+bool hipok(hipError_t e, const char *what) {
+  if (e != hipSuccess) {
+    fail(100, "%s: %s", what, hipGetErrorString(e));
+    return false;
+  }
+  return true;
+}
+
+template <typename T>
+T *dalloc(size_t n) {
+  void *p = nullptr;
+  if (n == 0) n = 1;
+  if (!hipok(hipMalloc(&p, n * sizeof(T)), "hipMalloc")) return nullptr;
+  hipMemset(p, 0, n * sizeof(T));
+  C().allocs.push_back(p);
+  return (T *)p;
+}
+
+void h2d(void *d, const void *h, size_t bytes) { hipok(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice), "h2d"); }
+void d2h(void *h, const void *d, size_t bytes) { hipok(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost), "d2h"); }
+
+void free_all() {
+  Ctx &c = C();
+  if (c.st) hipStreamSynchronize(c.st);
+  for (void *p : c.allocs) hipFree(p);
+  c.allocs.clear();
+  for (auto &kv : c.timers)
+    for (auto &e : kv.second.pending) {
+      hipEventDestroy(e.first);
+      hipEventDestroy(e.second);
+    }
+  c.timers.clear();
+  if (c.err_h) hipHostFree(c.err_h);
+  c.err_h = nullptr;
+}
+
+// ---------------------------------------------------------------- timing
+struct Scope {
+  const char *name;
+  hipEvent_t a = nullptr, b = nullptr;
+  explicit Scope(const char *n) : name(n) {
+    Ctx &c = C();
+    if (!c.prof) return;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    hipEventRecord(a, c.st);
+  }
+  ~Scope() {
+    Ctx &c = C();
+    if (!c.prof || !a) return;
+    hipEventRecord(b, c.st);
+    c.timers[name].pending.push_back({a, b});
+  }
+};
+
+void flush_timers() {
+  Ctx &c = C();
+  hipStreamSynchronize(c.st);
+  for (auto &kv : c.timers) {
+    for (auto &e : kv.second.pending) {
+      float ms = 0;
+      hipEventElapsedTime(&ms, e.first, e.second);
+      kv.second.total_ms += ms;
+      kv.second.count += 1;
+      hipEventDestroy(e.first);
+      hipEventDestroy(e.second);
+    }
+    kv.second.pending.clear();
+  }
+}
+
+// ------------------------------------------------------ connectivity
+// point lists of the 8 connections (ConnectivityHelpers.hpp), pt = igp*4+jgp
+const int EDGE_FWD[4][4] = {{0, 1, 2, 3}, {12, 13, 14, 15}, {0, 4, 8, 12}, {3, 7, 11, 15}};
+const int CORNER_PT[4] = {0, 3, 12, 15};
+const int EDGE_DIR[4][4] = {{1, 0, 0, 1}, {0, 1, 1, 0}, {0, 1, 1, 0}, {1, 0, 0, 1}};
+
+void build_dss_tables() {
+  Ctx &c = C();
+  const int ne = c.nelem;
+  // slot (receiver elem, receiver pos)[k] <- (sender elem, sender point)
+  std::vector<int> slot((size_t)ne * 8 * 4, -1);
+  for (int a = 0; a < ne; ++a)
+    for (int pa = 0; pa < 8; ++pa) {
+      const int b = c.conn_rl[a * 8 + pa];
+      if (b < 0) continue;
+      const int pb = c.conn_rp[a * 8 + pa];
+      if (pa < 4) {
+        const int dir = EDGE_DIR[pa][pb];
+        for (int k = 0; k < 4; ++k) {
+          const int spt = EDGE_FWD[pa][dir ? 3 - k : k];
+          slot[((size_t)b * 8 + pb) * 4 + k] = a * 16 + spt;
+        }
+      } else {
+        slot[((size_t)b * 8 + pb) * 4 + 0] = a * 16 + CORNER_PT[pa - 4];
+      }
+    }
+  std::vector<int> src((size_t)ne * 16 * 3, -1), cnt((size_t)ne * 16, 0);
+  std::vector<int> nbr((size_t)ne * 8, -1);
+  auto push = [&](int e, int pt, int s) {
+    const size_t i = (size_t)e * 16 + pt;
+    if (cnt[i] >= 3) {
+      fail(10, "more than three neighbour copies at element %d point %d", e, pt);
+      return;
+    }
+    src[i * 3 + cnt[i]++] = s;
+  };
+  for (int e = 0; e < ne; ++e) {
+    for (int k = 0; k < 4; ++k)
+      for (int ed = 0; ed < 4; ++ed) {
+        const int s = slot[((size_t)e * 8 + ed) * 4 + k];
+        if (s >= 0) push(e, EDGE_FWD[ed][k], s);
+      }
+    for (int cr = 0; cr < 4; ++cr) {
+      const int s = slot[((size_t)e * 8 + 4 + cr) * 4];
+      if (s >= 0) push(e, CORNER_PT[cr], s);
+    }
+    for (int pos = 0; pos < 8; ++pos) {
+      const int s = slot[((size_t)e * 8 + pos) * 4];
+      nbr[(size_t)e * 8 + pos] = s >= 0 ? s / 16 : -1;
+    }
+  }
+  c.dss_src = dalloc<int>(src.size());
+  c.nbr = dalloc<int>(nbr.size());
+  h2d(c.dss_src, src.data(), src.size() * sizeof(int));
+  h2d(c.nbr, nbr.data(), nbr.size() * sizeof(int));
+}
+
+MetricPtrs metric() {
+  Ctx &c = C();
+  MetricPtrs m;
+  m.D = c.D;
+  m.Dinv = c.Dinv;
+  m.metinv = c.metinv;
+  m.fcor = c.fcor;
+  m.mp = c.mp;
+  m.spheremp = c.spheremp;
+  m.rspheremp = c.rspheremp;
+  m.metdet = c.metdet;
+  m.phis = c.phis;
+  m.dvv = c.dvv_d;
+  return m;
+}
+
+bool ready(const char *who) {
+  Ctx &c = C();
+  if (!(c.p.set && c.dvv_set && c.elems_set && c.states_set && c.hv_set && c.conn_final)) {
+    fail(20, "%s: the library is not fully initialised (params %d, derivative %d, elements %d, states %d, hvcoord %d, "
+             "connectivity %d)",
+         who, c.p.set, c.dvv_set, c.elems_set, c.states_set, c.hv_set, c.conn_final);
+    return false;
+  }
+  if (c.conn_nelem != c.nelem) {
+    fail(21, "%s: connectivity has %d elements but the element data has %d", who, c.conn_nelem, c.nelem);
+    return false;
+  }
+  return true;
+}
+
+void alloc_work() {
+  Ctx &c = C();
+  if (c.be_set) return;
+  const size_t e3 = (size_t)c.nelem * c.nlev * 16;
+  const size_t q3 = e3 * (c.qsize ? c.qsize : 1);
+  c.cv = dalloc<double>(2 * e3);
+  c.ct = dalloc<double>(e3);
+  c.cdp = dalloc<double>(e3);
+  c.vt_a = dalloc<double>(2 * e3);
+  c.vt_b = dalloc<double>(2 * e3);
+  c.tt_a = dalloc<double>(e3);
+  c.tt_b = dalloc<double>(e3);
+  c.dpt_a = dalloc<double>(e3);
+  c.dpt_b = dalloc<double>(e3);
+  c.qdp_pre = dalloc<double>(q3);
+  c.qbih = dalloc<double>(q3);
+  c.qbih_pre = dalloc<double>(q3);
+  c.qlim[0] = dalloc<double>((size_t)c.nelem * (c.qsize ? c.qsize : 1) * 2 * c.nlev);
+  c.qlim[1] = dalloc<double>((size_t)c.nelem * (c.qsize ? c.qsize : 1) * 2 * c.nlev);
+  c.fdss_pre = dalloc<double>(e3);
+  c.err_d = dalloc<int>(1);
+  if (!c.err_h) hipHostMalloc((void **)&c.err_h, sizeof(int));
+  c.be_set = true;
+}
+
+// ------------------------------------------------------ step phases
+void caar_stage(int nm1, int n0, int np1, double dt, double eta_ave_w) {
+  Ctx &c = C();
+  CaarArgs a{};
+  a.m = metric();
+  a.nelem = c.nelem;
+  a.nlev = c.nlev;
+  a.qsize = c.qsize;
+  a.v_n0 = c.v[n0];
+  a.t_n0 = c.t[n0];
+  a.dp_n0 = c.dp[n0];
+  a.v_nm1 = c.v[nm1];
+  a.t_nm1 = c.t[nm1];
+  a.dp_nm1 = c.dp[nm1];
+  a.qdp_n0 = (c.p.moist && c.qsize > 0) ? c.qdp[c.n0_qdp] : nullptr;
+  a.vn0 = c.vn0;
+  a.omega = c.omega;
+  a.phi = c.phi;
+  a.out_v = c.cv;
+  a.out_t = c.ct;
+  a.out_dp = c.cdp;
+  a.dt = dt;
+  a.eta_ave_w = eta_ave_w;
+  a.p_top = c.hyai0 * c.ps0;
+  {
+    Scope s("caar");
+    launch_caar(c.st, a);
+  }
+  DssArgs x{};
+  x.nf = 4;
+  x.nelem = c.nelem;
+  x.nlev = c.nlev;
+  x.src = c.dss_src;
+  x.rsmp = c.rspheremp;
+  const long long es = (long long)c.nlev * 16, ev = (long long)c.nlev * 32;
+  x.f[0] = {c.cv, c.v[np1], ev, 32};
+  x.f[1] = {c.cv + 16, c.v[np1] + 16, ev, 32};
+  x.f[2] = {c.ct, c.t[np1], es, 16};
+  x.f[3] = {c.cdp, c.dp[np1], es, 16};
+  Scope s("caar_dss");
+  launch_dss(c.st, x);
+}
+
+bool stop_here(int phase) {
+  Ctx &c = C();
+  if (c.debug_stop == phase) c.stopped = true;
+  return c.stopped;
+}
+
+void u3_5stage(int nm1, int n0, int np1, double dt, double eta_ave_w) {
+  Ctx &c = C();
+  caar_stage(n0, n0, nm1, dt / 5.0, eta_ave_w / 4.0);
+  if (stop_here(1)) return;
+  caar_stage(n0, nm1, np1, dt / 5.0, 0.0);
+  caar_stage(n0, np1, np1, dt / 3.0, 0.0);
+  caar_stage(n0, np1, np1, 2.0 * dt / 3.0, 0.0);
+  {
+    Scope s("rk_combo");
+    launch_rk_combo(c.st, c.nelem, c.nlev, c.v[nm1], c.t[nm1], c.dp[nm1], c.v[n0], c.t[n0], c.dp[n0]);
+  }
+  caar_stage(nm1, np1, np1, 3.0 * dt / 4.0, 3.0 * eta_ave_w / 4.0);
+}
+
+void hyperviscosity(int np1, double dt, double eta_ave_w) {
+  Ctx &c = C();
+  HvArgs a{};
+  a.m = metric();
+  a.src = c.dss_src;
+  a.nelem = c.nelem;
+  a.nlev = c.nlev;
+  a.v = c.v[np1];
+  a.t = c.t[np1];
+  a.dp = c.dp[np1];
+  a.vt_a = c.vt_a;
+  a.tt_a = c.tt_a;
+  a.dpt_a = c.dpt_a;
+  a.vt_b = c.vt_b;
+  a.tt_b = c.tt_b;
+  a.dpt_b = c.dpt_b;
+  a.dpdiss_ave = c.dpdiss_ave;
+  a.dpdiss_bih = c.dpdiss_bih;
+  a.dt = dt / c.p.hypervis_subcycle;
+  a.eta_ave_w = eta_ave_w;
+  a.nu = c.p.nu;
+  a.nu_s = c.p.nu_s;
+  a.nu_p = c.p.nu_p;
+  a.nu_top = c.p.nu_top;
+  a.nu_ratio1 = c.p.nu_ratio1;
+  a.nu_ratio2 = c.p.nu_ratio2;
+  a.nsub = c.p.hypervis_subcycle;
+  for (int cyc = 0; cyc < c.p.hypervis_subcycle; ++cyc) {
+    Scope s("hv_cycle");
+    launch_hv_cycle(c.st, a);
+  }
+}
+
+void update_tracers_levels() {
+  Ctx &c = C();
+  const int it = c.nstep / c.p.qsplit;
+  c.n0_qdp = it % 2 == 0 ? 0 : 1;
+  c.np1_qdp = 1 - c.n0_qdp;
+}
+
+void leapfrog() {
+  Ctx &c = C();
+  const int tmp = c.np1;
+  c.np1 = c.nm1;
+  c.nm1 = c.n0;
+  c.n0 = tmp;
+  ++c.nstep;
+}
+
+void advec_tracers(double dt) {
+  Ctx &c = C();
+  update_tracers_levels();
+  EulerArgs a{};
+  a.m = metric();
+  a.src = c.dss_src;
+  a.nbr = c.nbr;
+  a.nelem = c.nelem;
+  a.nlev = c.nlev;
+  a.qsize = c.qsize;
+  a.qdp_pre = c.qdp_pre;
+  a.qbih = c.qbih;
+  a.qbih_pre = c.qbih_pre;
+  a.derived_dp = c.derived_dp;
+  a.divdp = c.divdp;
+  a.divdp_proj = c.divdp_proj;
+  a.divdp_w = c.divdp;
+  a.divdp_proj_w = c.divdp_proj;
+  a.vn0 = c.vn0;
+  a.dpdiss_ave = c.dpdiss_ave;
+  a.dpdiss_bih = c.dpdiss_bih;
+  a.dp0 = c.dp0_d;
+  a.fdss_pre = c.fdss_pre;
+  a.nu_p = c.p.nu_p;
+  a.nu_q = c.p.nu_q;
+  a.limiter_option = c.p.limiter_option;
+  a.rhs_viss = 0.0;
+  {
+    Scope s("euler_divdp");
+    launch_euler_divdp(c.st, a);
+  }
+  int cur = 0;
+  const double dts = dt / 2.0;
+  for (int stage = 0; stage < 3; ++stage) {
+    const int n0q = stage == 0 ? c.n0_qdp : c.np1_qdp;
+    a.qdp_n0 = c.qdp[n0q];
+    a.qdp_np1 = c.qdp[c.np1_qdp];
+    a.dt = dts;
+    a.rhs_multiplier = (double)stage;
+    a.qlim = c.qlim[cur];
+    a.qlim_new = c.qlim[1 - cur];
+    {
+      Scope s("euler_qminmax");
+      launch_euler_qminmax(c.st, a, stage == 2);
+    }
+    if (stage == 0) {
+      Scope s("euler_nbr_minmax");
+      launch_euler_nbr_minmax(c.st, a);
+      cur = 1 - cur;
+    } else if (stage == 2) {
+      a.rhs_viss = 3.0;
+      {
+        Scope s("euler_biharmonic");
+        launch_euler_biharmonic(c.st, a);
+      }
+      Scope s("euler_nbr_minmax");
+      launch_euler_nbr_minmax(c.st, a);
+      cur = 1 - cur;
+    }
+    a.qlim = c.qlim[cur];
+    a.qlim_new = c.qlim[1 - cur];
+    a.fdss = stage == 0 ? c.divdp_proj : stage == 1 ? c.eta_dot : c.omega;
+    a.qdp_avg_n0 = stage == 2 ? c.qdp[c.n0_qdp] : nullptr;
+    {
+      Scope s("euler_advect");
+      launch_euler_advect(c.st, a);
+    }
+    {
+      Scope s("euler_dss");
+      launch_euler_dss(c.st, a, stage == 2);
+    }
+    if (stage == 0 && stop_here(4)) return;
+  }
+}
+
+void prim_step(double dt) {
+  Ctx &c = C();
+  {
+    Scope s("prim_step_init");
+    launch_prim_step_init(c.st, c.nelem, c.nlev, c.p.nu_p > 0, c.dp[c.n0], c.eta_dot, c.vn0, c.omega, c.dpdiss_ave,
+                          c.dpdiss_bih, c.derived_dp);
+  }
+  for (int n = 0; n < c.p.qsplit; ++n) {
+    if (n > 0) leapfrog();
+    if (c.p.moist) update_tracers_levels();
+    const double eta_ave_w = 1.0 / c.p.qsplit;
+    u3_5stage(c.nm1, c.n0, c.np1, dt, eta_ave_w);
+    if (c.stopped || stop_here(2)) return;
+    hyperviscosity(c.np1, dt, eta_ave_w);
+    if (stop_here(3)) return;
+  }
+  if (c.qsize > 0) advec_tracers(dt * c.p.qsplit);
+  stop_here(5);
+}
+
+void remap(double dt_remap) {
+  Ctx &c = C();
+  (void)dt_remap;
+  RemapArgs a{};
+  a.nelem = c.nelem;
+  a.nlev = c.nlev;
+  a.qsize = c.qsize;
+  a.remap_alg = c.p.remap_alg;
+  a.v = c.v[c.np1];
+  a.t = c.t[c.np1];
+  a.dp = c.dp[c.np1];
+  a.ps = c.ps[c.np1];
+  a.qdp = c.qsize ? c.qdp[c.np1_qdp] : nullptr;
+  a.Q = c.Q;
+  a.hyai = c.hyai_d;
+  a.hybi = c.hybi_d;
+  a.ps0 = c.ps0;
+  a.hyai0 = c.hyai0;
+  a.error = c.err_d;
+  Scope s("remap");
+  launch_remap(c.st, a);
+}
+
+// Fortran (np,np,nlev,...) blocks are already [lev][pt] per element; only the
+// time-level / tracer strides differ between the Fortran arrays and ours.
+void pull_levels(double *dev, const double *f90, int blocks_per_elem_f, int which, size_t block) {
+  Ctx &c = C();
+  std::vector<double> tmp((size_t)c.nelem * block);
+  for (int ie = 0; ie < c.nelem; ++ie)
+    memcpy(&tmp[(size_t)ie * block], f90 + ((size_t)ie * blocks_per_elem_f + which) * block, block * sizeof(double));
+  h2d(dev, tmp.data(), tmp.size() * sizeof(double));
+}
+void push_levels(double *f90, const double *dev, int blocks_per_elem_f, int which, size_t block) {
+  Ctx &c = C();
+  std::vector<double> tmp((size_t)c.nelem * block);
+  d2h(tmp.data(), dev, tmp.size() * sizeof(double));
+  for (int ie = 0; ie < c.nelem; ++ie)
+    memcpy(f90 + ((size_t)ie * blocks_per_elem_f + which) * block, &tmp[(size_t)ie * block], block * sizeof(double));
+}
+
+// tracer arrays: Fortran [ie][tl][qsize_d][lev][pt] vs device [ie][qsize][lev][pt]
+void pull_tracers(double *dev, const double *f90, int ntl_f, int tl) {
+  Ctx &c = C();
+  const size_t e3 = (size_t)c.nlev * 16;
+  std::vector<double> tmp((size_t)c.nelem * c.qsize * e3);
+  for (int ie = 0; ie < c.nelem; ++ie)
+    for (int q = 0; q < c.qsize; ++q)
+      memcpy(&tmp[((size_t)ie * c.qsize + q) * e3], f90 + (((size_t)ie * ntl_f + tl) * c.qsize_d + q) * e3,
+             e3 * sizeof(double));
+  if (!tmp.empty()) h2d(dev, tmp.data(), tmp.size() * sizeof(double));
+}
+void push_tracers(double *f90, const double *dev, int ntl_f, int tl) {
+  Ctx &c = C();
+  const size_t e3 = (size_t)c.nlev * 16;
+  std::vector<double> tmp((size_t)c.nelem * c.qsize * e3);
+  if (!tmp.empty()) d2h(tmp.data(), dev, tmp.size() * sizeof(double));
+  for (int ie = 0; ie < c.nelem; ++ie)
+    for (int q = 0; q < c.qsize; ++q)
+      memcpy(f90 + (((size_t)ie * ntl_f + tl) * c.qsize_d + q) * e3, &tmp[((size_t)ie * c.qsize + q) * e3],
+             e3 * sizeof(double));
+}
+
+}  // namespace
+
+// =================================================================== C-ABI
+extern "C" {
+
+// Hommexx_Session.cpp:35
+void initialize_hommexx_session() {
+  Ctx &c = C();
+  if (c.session) return;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    fail(1, "initialize_hommexx_session: no HIP device");
+    return;
+  }
+  if (!hipok(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking), "hipStreamCreate")) return;
+  c.session = true;
+}
+
+// Hommexx_Session.cpp:75
+void finalize_hommexx_session() {
+  if (!g) return;
+  free_all();
+  if (g->st) hipStreamDestroy(g->st);
+  const bool abort_flag = g->abort_on_error;
+  delete g;
+  g = nullptr;
+  C().abort_on_error = abort_flag;
+}
+
+// mpi/mpi_cxx_f90_interface.cpp:20 (one rank: the communicator is recorded only)
+void reset_cxx_comm(const int &f_comm) { (void)f_comm; }
+
+// mpi/mpi_cxx_f90_interface.cpp:27
+void init_connectivity(const int &num_local_elems) {
+  Ctx &c = C();
+  if (num_local_elems < 0) {
+    fail(30, "init_connectivity: negative element count");
+    return;
+  }
+  c.conn_nelem = num_local_elems;
+  c.conn_final = false;
+  c.conn_rl.assign((size_t)num_local_elems * 8, -1);
+  c.conn_rp.assign((size_t)num_local_elems * 8, -1);
+}
+
+// mpi/mpi_cxx_f90_interface.cpp:34-55 (1-based ids, Fortran positions W,E,S,N,SW,SE,NW,NE)
+void add_connection(const int &first_lid, const int &first_gid, const int &first_pos, const int &first_pid,
+                    const int &second_lid, const int &second_gid, const int &second_pos, const int &second_pid) {
+  Ctx &c = C();
+  (void)first_gid;
+  (void)second_gid;
+  if (first_lid <= 0 || first_gid <= 0 || first_pos <= 0 || first_pid <= 0 || second_lid <= 0 || second_gid <= 0 ||
+      second_pos <= 0 || second_pid <= 0) {
+    fail(31, "add_connection: Fortran indices must start at 1");
+    return;
+  }
+  if (first_pid != 1) return;  // only connections owned by this (single) rank are stored
+  if (second_pid != 1) {
+    fail(32, "add_connection: remote ranks are not supported on one GPU");
+    return;
+  }
+  auto conv = [](int f) { return f <= 4 ? ((f - 1) + 2) % 4 : f - 1; };
+  const int p1 = conv(first_pos), p2 = conv(second_pos);
+  if (p1 > 7 || p2 > 7 || (p1 < 4) != (p2 < 4) || first_lid > c.conn_nelem || second_lid > c.conn_nelem) {
+    fail(33, "add_connection: invalid connection (%d,%d)->(%d,%d)", first_lid, first_pos, second_lid, second_pos);
+    return;
+  }
+  c.conn_rl[(size_t)(first_lid - 1) * 8 + p1] = second_lid - 1;
+  c.conn_rp[(size_t)(first_lid - 1) * 8 + p1] = p2;
+}
+
+// mpi/mpi_cxx_f90_interface.cpp:57
+void finalize_connectivity() {
+  Ctx &c = C();
+  for (int ie = 0; ie < c.conn_nelem; ++ie) {
+    int missing = 0;
+    for (int p = 0; p < 8; ++p) {
+      if (c.conn_rl[(size_t)ie * 8 + p] < 0) {
+        if (p < 4) {
+          fail(34, "finalize_connectivity: element %d has no edge neighbour at position %d", ie + 1, p);
+          return;
+        }
+        ++missing;
+      }
+    }
+    if (missing > 1) {
+      fail(35, "finalize_connectivity: element %d misses %d corner neighbours", ie + 1, missing);
+      return;
+    }
+  }
+  c.conn_final = true;
+}
+
+// cxx_f90_interface.cpp:207
+void init_derivative_c(const double *const &dvv) {
+  Ctx &c = C();
+  memcpy(c.dvv, dvv, sizeof(c.dvv));
+  c.dvv_set = true;
+}
+
+// cxx_f90_interface.cpp:33-117
+void init_simulation_params_c(const int &remap_alg, const int &limiter_option, const int &rsplit, const int &qsplit,
+                              const int &time_step_type, const int &energy_fixer, const int &qsize,
+                              const int &state_frequency, const double &nu, const double &nu_p, const double &nu_q,
+                              const double &nu_s, const double &nu_div, const double &nu_top,
+                              const int &hypervis_order, const int &hypervis_subcycle, const double &hypervis_scaling,
+                              const int &ftype, const bool &prescribed_wind, const bool &moisture,
+                              const bool &disable_diagnostics, const bool &use_cpstar,
+                              const bool &use_semi_lagrangian_transport) {
+  Ctx &c = C();
+  auto bad = [](const char *opt, const char *val) {
+    fail(40, "init_simulation_params_c: unsupported option %s = %s", opt, val);
+  };
+  char buf[64];
+  if (remap_alg != 1 && remap_alg != 2) return snprintf(buf, 64, "%d", remap_alg), bad("vert_remap_q_alg", buf);
+  if (prescribed_wind) return bad("prescribed_wind", "true");
+  if (hypervis_order != 2) return snprintf(buf, 64, "%d", hypervis_order), bad("hypervis_order", buf);
+  if (use_semi_lagrangian_transport) return bad("use_semi_lagrangian_transport", "true");
+  if (time_step_type != 5) return snprintf(buf, 64, "%d", time_step_type), bad("time_step_type", buf);
+  if (limiter_option != 8 && limiter_option != 9)
+    return snprintf(buf, 64, "%d", limiter_option), bad("limiter_option", buf);
+  if (ftype != -1 && ftype != 0 && ftype != 2) return snprintf(buf, 64, "%d", ftype), bad("ftype", buf);
+  if (!(nu_p > 0)) return snprintf(buf, 64, "%g", nu_p), bad("nu_p", buf);
+  if (!(nu > 0)) return snprintf(buf, 64, "%g", nu), bad("nu", buf);
+  if (!(nu_div > 0)) return snprintf(buf, 64, "%g", nu_div), bad("nu_div", buf);
+  if (hypervis_scaling != 0.0) return snprintf(buf, 64, "%g", hypervis_scaling), bad("hypervis_scaling", buf);
+  if (rsplit < 1) return snprintf(buf, 64, "%d", rsplit), bad("rsplit", buf);
+  if (qsplit < 1 || hypervis_subcycle < 1 || qsize < 0)
+    return bad("qsplit/hypervis_subcycle/qsize", "out of range");
+  Params &p = c.p;
+  p.remap_alg = remap_alg;
+  p.limiter_option = limiter_option;
+  p.rsplit = rsplit;
+  p.qsplit = qsplit;
+  p.time_step_type = time_step_type;
+  p.energy_fixer = energy_fixer;
+  p.qsize = qsize;
+  p.state_frequency = state_frequency;
+  p.nu = nu;
+  p.nu_p = nu_p;
+  p.nu_q = nu_q;
+  p.nu_s = nu_s;
+  p.nu_div = nu_div;
+  p.nu_top = nu_top;
+  p.hypervis_order = hypervis_order;
+  p.hypervis_subcycle = hypervis_subcycle;
+  p.hypervis_scaling = hypervis_scaling;
+  p.ftype = ftype;
+  p.prescribed_wind = prescribed_wind;
+  p.moist = moisture;
+  p.disable_diagnostics = disable_diagnostics;
+  p.use_cpstar = use_cpstar;
+  p.use_sl = use_semi_lagrangian_transport;
+  if (p.nu != p.nu_div) {
+    const double ratio = p.nu_div / p.nu;
+    if (p.hypervis_scaling != 0.0) {
+      p.nu_ratio1 = ratio * ratio;
+      p.nu_ratio2 = 1.0;
+    } else {
+      p.nu_ratio1 = ratio;
+      p.nu_ratio2 = ratio;
+    }
+  } else {
+    p.nu_ratio1 = 1.0;
+    p.nu_ratio2 = 1.0;
+  }
+  p.set = true;
+}
+
+// cxx_f90_interface.cpp:119 (hyam/hybm are accepted but unused, as in the reference)
+void init_hvcoord_c(const double &ps0, const double *const &hybrid_am, const double *const &hybrid_ai,
+                    const double *const &hybrid_bm, const double *const &hybrid_bi) {
+  Ctx &c = C();
+  (void)hybrid_am;
+  (void)hybrid_bm;
+  if (c.nlev <= 0) c.nlev = c.nlev_cfg;
+  const int L = c.nlev;
+  c.ps0 = ps0;
+  c.hyai.assign(hybrid_ai, hybrid_ai + L + 1);
+  c.hybi.assign(hybrid_bi, hybrid_bi + L + 1);
+  c.hyai0 = hybrid_ai[0];
+  std::vector<double> dp0(L);
+  for (int k = 0; k < L; ++k) {
+    const double da = c.hyai[k + 1] - c.hyai[k], db = c.hybi[k + 1] - c.hybi[k];
+    dp0[k] = da * ps0 + db * ps0;
+  }
+  if (!c.hyai_d) {
+    c.hyai_d = dalloc<double>(L + 1);
+    c.hybi_d = dalloc<double>(L + 1);
+    c.dp0_d = dalloc<double>(L);
+  }
+  h2d(c.hyai_d, c.hyai.data(), (L + 1) * sizeof(double));
+  h2d(c.hybi_d, c.hybi.data(), (L + 1) * sizeof(double));
+  h2d(c.dp0_d, dp0.data(), L * sizeof(double));
+  c.hv_set = true;
+}
+
+// cxx_f90_interface.cpp:224-233
+void init_elements_2d_c(const int &num_elems, const double *const &D, const double *const &Dinv,
+                        const double *const &fcor, const double *const &mp, const double *const &spheremp,
+                        const double *const &rspheremp, const double *const &metdet, const double *const &metinv,
+                        const double *const &phis, const double *const &tensorvisc,
+                        const double *const &vec_sph2cart, const bool &consthv) {
+  Ctx &c = C();
+  if (!c.session) initialize_hommexx_session();
+  if (!c.p.set) {
+    fail(50, "init_elements_2d_c: init_simulation_params_c must come first");
+    return;
+  }
+  if (c.elems_set) {
+    fail(51, "init_elements_2d_c: elements already initialised");
+    return;
+  }
+  c.nelem = num_elems;
+  c.nlev = c.nlev_cfg;
+  c.qsize = c.p.qsize;
+  c.qsize_d = c.qsize_d_cfg > 0 ? c.qsize_d_cfg : std::max(c.qsize, 1);
+  if (c.qsize_d < c.qsize) {
+    fail(52, "init_elements_2d_c: qsize %d exceeds qsize_d %d", c.qsize, c.qsize_d);
+    return;
+  }
+  const size_t ne = num_elems;
+  auto up = [&](double *&dst, const double *src, size_t n) {
+    dst = dalloc<double>(n);
+    if (src) h2d(dst, src, n * sizeof(double));
+  };
+  up(c.D, D, ne * 64);
+  up(c.Dinv, Dinv, ne * 64);
+  up(c.metinv, metinv, ne * 64);
+  up(c.fcor, fcor, ne * 16);
+  up(c.mp, mp, ne * 16);
+  up(c.spheremp, spheremp, ne * 16);
+  up(c.rspheremp, rspheremp, ne * 16);
+  up(c.metdet, metdet, ne * 16);
+  up(c.phis, phis, ne * 16);
+  c.consthv = consthv;
+  if (!consthv) {
+    up(c.tensorvisc, tensorvisc, ne * 64);
+    up(c.vec_sph2cart, vec_sph2cart, ne * 96);
+  }
+  c.dvv_d = dalloc<double>(16);
+  h2d(c.dvv_d, c.dvv, sizeof(c.dvv));
+  const size_t e3 = ne * c.nlev * 16;
+  for (int tl = 0; tl < 3; ++tl) {
+    c.v[tl] = dalloc<double>(2 * e3);
+    c.t[tl] = dalloc<double>(e3);
+    c.dp[tl] = dalloc<double>(e3);
+    c.ps[tl] = dalloc<double>(ne * 16);
+  }
+  for (int tl = 0; tl < 2; ++tl) c.qdp[tl] = dalloc<double>(e3 * (c.qsize ? c.qsize : 1));
+  c.Q = dalloc<double>(e3 * (c.qsize ? c.qsize : 1));
+  c.phi = dalloc<double>(e3);
+  c.omega = dalloc<double>(e3);
+  c.eta_dot = dalloc<double>(e3);
+  c.dpdiss_ave = dalloc<double>(e3);
+  c.dpdiss_bih = dalloc<double>(e3);
+  c.derived_dp = dalloc<double>(e3);
+  c.divdp = dalloc<double>(e3);
+  c.divdp_proj = dalloc<double>(e3);
+  c.vn0 = dalloc<double>(2 * e3);
+  c.ft = dalloc<double>(e3);
+  c.fm = dalloc<double>(2 * e3);
+  c.fq = dalloc<double>(e3 * (c.qsize ? c.qsize : 1));
+  c.elems_set = true;
+}
+
+// cxx_f90_interface.cpp:235-251
+void init_elements_states_c(const double *const &v, const double *const &T, const double *const &dp3d,
+                            const double *const &qdp, const double *const &ps_v) {
+  Ctx &c = C();
+  if (!c.elems_set) {
+    fail(53, "init_elements_states_c: init_elements_2d_c must come first");
+    return;
+  }
+  const size_t e3 = (size_t)c.nlev * 16;
+  for (int tl = 0; tl < 3; ++tl) {
+    pull_levels(c.v[tl], v, 3, tl, 2 * e3);
+    pull_levels(c.t[tl], T, 3, tl, e3);
+    pull_levels(c.dp[tl], dp3d, 3, tl, e3);
+    pull_levels(c.ps[tl], ps_v, 3, tl, 16);
+  }
+  for (int tl = 0; tl < 2; ++tl) pull_tracers(c.qdp[tl], qdp, 2, tl);
+  // tracers beyond qsize are carried unchanged (the reference keeps them in its qdp view)
+  c.qdp_tail.clear();
+  if (c.qsize_d > c.qsize) {
+    const size_t tail = (size_t)(c.qsize_d - c.qsize) * e3;
+    c.qdp_tail.resize((size_t)c.nelem * 2 * tail);
+    for (int ie = 0; ie < c.nelem; ++ie)
+      for (int tl = 0; tl < 2; ++tl)
+        memcpy(&c.qdp_tail[((size_t)ie * 2 + tl) * tail], qdp + (((size_t)ie * 2 + tl) * c.qsize_d + c.qsize) * e3,
+               tail * sizeof(double));
+  }
+  c.states_set = true;
+}
+
+// cxx_f90_interface.cpp:253-262 (diagnostics are disabled on this path)
+void init_diagnostics_c(double *&, double *&, double *&, double *&, double *&, double *&, double *&, double *&) {}
+
+// cxx_f90_interface.cpp:264-280
+void init_boundary_exchanges_c() {
+  Ctx &c = C();
+  if (!c.elems_set || !c.conn_final) {
+    fail(54, "init_boundary_exchanges_c: elements and connectivity must be initialised first");
+    return;
+  }
+  if (c.conn_nelem != c.nelem) {
+    fail(21, "init_boundary_exchanges_c: connectivity has %d elements, element data %d", c.conn_nelem, c.nelem);
+    return;
+  }
+  if (!c.dss_src) build_dss_tables();
+  alloc_work();
+}
+
+// cxx_f90_interface.cpp:213-222 (1-based in)
+void init_time_level_c(const int &nm1, const int &n0, const int &np1, const int &nstep, const int &nstep0) {
+  Ctx &c = C();
+  c.nm1 = nm1 - 1;
+  c.n0 = n0 - 1;
+  c.np1 = np1 - 1;
+  c.nstep = nstep;
+  c.nstep0 = nstep0;
+}
+
+// prim_driver.cpp:31-156: one remap interval.  Indices come back 0-based.
+void prim_run_subcycle_c(const double &dt, int &nstep, int &nm1, int &n0, int &np1, const int &last_time_step) {
+  (void)last_time_step;
+  Ctx &c = C();
+  if (!ready("prim_run_subcycle_c")) return;
+  if (!c.be_set) init_boundary_exchanges_c();
+  const double dt_q = dt * c.p.qsplit;
+  const double dt_remap = c.p.rsplit > 0 ? dt_q * c.p.rsplit : dt_q;
+  update_tracers_levels();
+  if (c.p.ftype == 0) {
+    Scope s("forcing");
+    launch_state_forcing(c.st, c.nelem, c.nlev, dt_remap, c.ft, c.fm, c.t[c.n0], c.v[c.n0]);
+    launch_tracer_forcing(c.st, c.nelem, c.nlev, c.qsize, c.p.moist, dt_remap, c.fq, c.qdp[c.n0_qdp], c.Q, c.hyai_d,
+                          c.hybi_d, c.ps0, c.ps[c.n0]);
+  } else if (c.p.ftype == 2) {
+    Scope s("forcing");
+    launch_state_forcing(c.st, c.nelem, c.nlev, dt_remap, c.ft, c.fm, c.t[c.n0], c.v[c.n0]);
+  }
+  {
+    Scope s("dp3d_from_ps");
+    launch_dp3d_from_ps(c.st, c.nelem, c.nlev, c.hyai_d, c.hybi_d, c.ps0, c.ps[c.n0], c.dp[c.n0]);
+  }
+  c.stopped = false;
+  prim_step(dt);
+  for (int r = 1; r < c.p.rsplit && !c.stopped; ++r) {
+    leapfrog();
+    prim_step(dt);
+  }
+  if (c.stopped) {
+    hipStreamSynchronize(c.st);
+    return;
+  }
+  update_tracers_levels();
+  launch_zero_int(c.st, c.err_d);
+  remap(dt_remap);
+  leapfrog();
+  nstep = c.nstep;
+  nm1 = c.nm1;
+  n0 = c.n0;
+  np1 = c.np1;
+  hipMemcpyAsync(c.err_h, c.err_d, sizeof(int), hipMemcpyDeviceToHost, c.st);
+  if (!hipok(hipStreamSynchronize(c.st), "prim_run_subcycle_c")) return;
+  if (*c.err_h & 1) fail(60, "Negative (or nan) layer thickness detected, aborting!");
+  else if (*c.err_h & 2) fail(61, "vertical remap: Lagrangian levels deformed by more than one layer");
+}
+
+// cxx_f90_interface.cpp:126-154
+void cxx_push_results_to_f90(double *&v, double *&T, double *&dp3d, double *&Qdp, double *&Q, double *&ps_v,
+                             double *&omega_p) {
+  Ctx &c = C();
+  if (!c.states_set) {
+    fail(55, "cxx_push_results_to_f90: no state");
+    return;
+  }
+  hipStreamSynchronize(c.st);
+  const size_t e3 = (size_t)c.nlev * 16;
+  for (int tl = 0; tl < 3; ++tl) {
+    push_levels(v, c.v[tl], 3, tl, 2 * e3);
+    push_levels(T, c.t[tl], 3, tl, e3);
+    push_levels(dp3d, c.dp[tl], 3, tl, e3);
+    push_levels(ps_v, c.ps[tl], 3, tl, 16);
+  }
+  for (int tl = 0; tl < 2; ++tl) push_tracers(Qdp, c.qdp[tl], 2, tl);
+  if (c.qsize_d > c.qsize && !c.qdp_tail.empty()) {
+    const size_t tail = (size_t)(c.qsize_d - c.qsize) * e3;
+    for (int ie = 0; ie < c.nelem; ++ie)
+      for (int tl = 0; tl < 2; ++tl)
+        memcpy(Qdp + (((size_t)ie * 2 + tl) * c.qsize_d + c.qsize) * e3, &c.qdp_tail[((size_t)ie * 2 + tl) * tail],
+               tail * sizeof(double));
+  }
+  push_tracers(Q, c.Q, 1, 0);
+  for (int ie = 0; ie < c.nelem; ++ie)
+    for (int q = c.qsize; q < c.qsize_d; ++q) memset(Q + ((size_t)ie * c.qsize_d + q) * e3, 0, e3 * sizeof(double));
+  push_levels(omega_p, c.omega, 1, 0, e3);
+}
+
+// cxx_f90_interface.cpp:157-178
+void cxx_push_forcing_to_f90(double *FM, double *FT, double *FQ) {
+  Ctx &c = C();
+  hipStreamSynchronize(c.st);
+  const size_t e3 = (size_t)c.nlev * 16;
+  push_levels(FM, c.fm, 1, 0, 2 * e3);
+  push_levels(FT, c.ft, 1, 0, e3);
+  if (c.p.ftype == 0 && FQ) push_tracers(FQ, c.fq, 1, 0);
+}
+
+// cxx_f90_interface.cpp:180-205
+void f90_push_forcing_to_cxx(double *FM, double *FT, double *FQ, double *Qdp) {
+  Ctx &c = C();
+  const size_t e3 = (size_t)c.nlev * 16;
+  hipStreamSynchronize(c.st);
+  pull_levels(c.fm, FM, 1, 0, 2 * e3);
+  pull_levels(c.ft, FT, 1, 0, e3);
+  if (c.p.ftype == 0 && FQ) pull_tracers(c.fq, FQ, 1, 0);
+  for (int tl = 0; tl < 2; ++tl) pull_tracers(c.qdp[tl], Qdp, 2, tl);
+}
+
+// ============================================================ extensions
+// Dimensions the reference fixes at compile time (PLEV, QSIZE_D); call before
+// init_elements_2d_c.  Defaults: nlev = 72, qsize_d = qsize.
+int hommexx_hip_configure(int nlev, int qsize_d) {
+  Ctx &c = C();
+  if (c.elems_set) return 1;
+  if (nlev < 3 || nlev > 4096) return 2;
+  c.nlev_cfg = nlev;
+  c.qsize_d_cfg = qsize_d;
+  return 0;
+}
+
+void hommexx_hip_set_abort_on_error(int on) { C().abort_on_error = on != 0; }
+// Debugging aid: make prim_run_subcycle_c return after phase n (1: first RK
+// stage, 2: all RK stages, 3: hyperviscosity, 4: first tracer stage,
+// 5: tracers); 0 disables.  Time levels are not rotated when it stops.
+void hommexx_hip_debug_stop(int phase) { C().debug_stop = phase; }
+int hommexx_hip_last_error() { return C().last_error; }
+const char *hommexx_hip_error_message() { return C().msg.c_str(); }
+void hommexx_hip_clear_error() {
+  C().last_error = 0;
+  C().msg.clear();
+}
+void hommexx_hip_synchronize() {
+  if (C().st) hipStreamSynchronize(C().st);
+}
+void *hommexx_hip_stream() { return (void *)C().st; }
+
+// Per-phase device timing with HIP events on the library's stream.
+void hommexx_hip_profile(int enable) {
+  Ctx &c = C();
+  if (c.prof && !enable) flush_timers();
+  c.prof = enable != 0;
+}
+void hommexx_hip_profile_reset() {
+  Ctx &c = C();
+  flush_timers();
+  c.timers.clear();
+}
+int hommexx_hip_profile_get(const char *name, int *count, double *total_ms) {
+  Ctx &c = C();
+  flush_timers();
+  auto it = c.timers.find(name);
+  if (it == c.timers.end()) {
+    *count = 0;
+    *total_ms = 0;
+    return 1;
+  }
+  *count = it->second.count;
+  *total_ms = it->second.total_ms;
+  return 0;
+}
+// names of the timed phases, '\n'-separated, into buf (returns needed length)
+int hommexx_hip_profile_names(char *buf, int len) {
+  Ctx &c = C();
+  flush_timers();
+  std::string s;
+  for (auto &kv : c.timers) {
+    s += kv.first;
+    s += "\n";
+  }
+  if (buf && len > 0) {
+    strncpy(buf, s.c_str(), len - 1);
+    buf[len - 1] = 0;
+  }
+  return (int)s.size() + 1;
+}
+
+// Current time-level indices (0-based) and sizes.
+void hommexx_hip_time_levels(int *nm1, int *n0, int *np1, int *nstep, int *n0_qdp, int *np1_qdp) {
+  Ctx &c = C();
+  *nm1 = c.nm1;
+  *n0 = c.n0;
+  *np1 = c.np1;
+  *nstep = c.nstep;
+  *n0_qdp = c.n0_qdp;
+  *np1_qdp = c.np1_qdp;
+}
+
+}  // extern "C"

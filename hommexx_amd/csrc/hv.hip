@@ -1,0 +1,127 @@
+// Biharmonic hyperviscosity on the np1 state, one subcycle per call
+// (HyperviscosityFunctorImpl.cpp:56-112, HyperviscosityFunctorImpl.hpp:71-257).
+// Slab kernels: a 16-lane group = one element at one level.
+//   lap1      : weak laplacians of T, dp3d and the vector laplacian of v
+//   DSS(*rsmp)
+//   lap2+pre  : second laplacians, dpdiss accumulators, -nu scaling, the
+//               nu_top sponge on the top three levels, dptens <- dp3d*sph + dt*dptens
+//   DSS
+//   update    : apply the inverse mass, v += , T += - heating/cp, dp3d = dptens*rsmp
+#include "kernels.hpp"
+
+namespace hx {
+
+__device__ __forceinline__ bool slab_of(int nblk, int nelem, int nlev, int &ie, int &k, int &pt) {
+  const int b = xcd_block(blockIdx.x, nblk);
+  const size_t slab = (size_t)b * 16 + (threadIdx.x >> 4);
+  pt = threadIdx.x & 15;
+  if (slab >= (size_t)nelem * nlev) return false;
+  ie = (int)(slab / nlev);
+  k = (int)(slab % nlev);
+  return true;
+}
+
+__global__ __launch_bounds__(256) void hv_lap1_kernel(HvArgs a) {
+  int ie, k, pt;
+  if (!slab_of(gridDim.x, a.nelem, a.nlev, ie, k, pt)) return;
+  ElemLane e;
+  load_elem(a.m, ie, pt, e);
+  DvvLane d;
+  load_dvv(a.m.dvv, pt, d);
+  const size_t s = ((size_t)ie * a.nlev + k) * 16 + pt;
+  const size_t v = ((size_t)ie * a.nlev + k) * 32 + pt;
+  a.tt_a[s] = sph_laplace(e, d, a.t[s]);
+  a.dpt_a[s] = sph_laplace(e, d, a.dp[s]);
+  double l0, l1;
+  sph_vlaplace_contra(e, d, a.nu_ratio1, a.v[v], a.v[v + 16], l0, l1);
+  a.vt_a[v] = l0;
+  a.vt_a[v + 16] = l1;
+}
+
+__global__ __launch_bounds__(256) void hv_lap2_pre_kernel(HvArgs a) {
+  int ie, k, pt;
+  if (!slab_of(gridDim.x, a.nelem, a.nlev, ie, k, pt)) return;
+  ElemLane e;
+  load_elem(a.m, ie, pt, e);
+  DvvLane d;
+  load_dvv(a.m.dvv, pt, d);
+  const size_t s = ((size_t)ie * a.nlev + k) * 16 + pt;
+  const size_t v = ((size_t)ie * a.nlev + k) * 32 + pt;
+  // second laplacian (consthv)
+  double tt = sph_laplace(e, d, a.tt_b[s]);
+  double dpt = sph_laplace(e, d, a.dpt_b[s]);
+  double vt0, vt1;
+  sph_vlaplace_contra(e, d, a.nu_ratio2, a.vt_b[v], a.vt_b[v + 16], vt0, vt1);
+  // pre-exchange
+  const double dpn = a.dp[s];
+  a.dpdiss_ave[s] += a.eta_ave_w * dpn / a.nsub;
+  a.dpdiss_bih[s] += a.eta_ave_w * dpt / a.nsub;
+  double lv0 = 0, lv1 = 0, lt = 0, ldp = 0;
+  const bool top = a.nu_top > 0 && k < 3;
+  if (top) {  // uniform over the slab
+    sph_vlaplace_contra(e, d, 1.0, a.v[v], a.v[v + 16], lv0, lv1);
+    lt = sph_laplace(e, d, a.t[s]);
+    ldp = sph_laplace(e, d, dpn);
+  }
+  vt0 *= -a.nu;
+  vt1 *= -a.nu;
+  tt *= -a.nu_s;
+  dpt *= -a.nu_p;
+  if (top) {
+    const double sc = (k == 0 ? 4.0 : k == 1 ? 2.0 : 1.0) * a.nu_top;
+    vt0 += sc * lv0;
+    vt1 += sc * lv1;
+    tt += sc * lt;
+    dpt += sc * ldp;
+  }
+  dpt *= a.dt;
+  dpt += dpn * e.sph;
+  a.vt_a[v] = vt0;
+  a.vt_a[v + 16] = vt1;
+  a.tt_a[s] = tt;
+  a.dpt_a[s] = dpt;
+}
+
+__global__ __launch_bounds__(256) void hv_update_kernel(HvArgs a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)a.nelem * a.nlev * 16) return;
+  const size_t row = i >> 4;
+  const int pt = i & 15;
+  const int ie = (int)(row / a.nlev);
+  const double r = a.m.rspheremp[(size_t)ie * 16 + pt];
+  const size_t v0 = row * 32 + pt, v1 = v0 + 16;
+  const double vt0 = (a.dt * a.vt_b[v0] * r);
+  const double vt1 = (a.dt * a.vt_b[v1] * r);
+  const double u = a.v[v0] + vt0, w = a.v[v1] + vt1;
+  a.v[v0] = u;
+  a.v[v1] = w;
+  const double tt = (a.dt * a.tt_b[i] * r);
+  const double heating = vt0 * u + vt1 * w;
+  a.t[i] = a.t[i] + tt - heating / CP;
+  a.dp[i] = (a.dpt_b[i] * r);
+}
+
+void launch_hv_cycle(hipStream_t st, const HvArgs &a) {
+  const size_t nslab = (size_t)a.nelem * a.nlev;
+  const int blocks = (int)((nslab + 15) / 16);
+  hipLaunchKernelGGL(hv_lap1_kernel, dim3(blocks), dim3(256), 0, st, a);
+  DssArgs x{};
+  x.nf = 4;
+  x.nelem = a.nelem;
+  x.nlev = a.nlev;
+  x.src = a.src;
+  x.rsmp = a.m.rspheremp;
+  const long long es = (long long)a.nlev * 16, ev = (long long)a.nlev * 32;
+  x.f[0] = {a.vt_a, a.vt_b, ev, 32};
+  x.f[1] = {a.vt_a + 16, a.vt_b + 16, ev, 32};
+  x.f[2] = {a.tt_a, a.tt_b, es, 16};
+  x.f[3] = {a.dpt_a, a.dpt_b, es, 16};
+  launch_dss(st, x);
+  hipLaunchKernelGGL(hv_lap2_pre_kernel, dim3(blocks), dim3(256), 0, st, a);
+  x.rsmp = nullptr;
+  launch_dss(st, x);
+  const size_t n = nslab * 16;
+  hipLaunchKernelGGL(hv_update_kernel, dim3((n + 255) / 256), dim3(256), 0, st, a);
+}
+
+}  // namespace hx

@@ -1,0 +1,108 @@
+// Kernel argument blocks and host-side launchers of the preqx step.
+// Device layouts (all FP64, one array per time level):
+//   scalar 3-D   [ie][lev][pt]            pt = igp*4 + jgp (Fortran (np,np,nlev))
+//   vector 3-D   [ie][lev][comp][pt]      (Fortran (np,np,2,nlev))
+//   tracers      [ie][q][lev][pt]         (Fortran (np,np,nlev,qsize))
+//   2-D scalar   [ie][pt],  2-D tensor [ie][idim*2+jdim][pt]
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sphere.hpp"
+
+namespace hx {
+
+struct CaarArgs {
+  MetricPtrs m;
+  int nelem, nlev, qsize;
+  const double *v_n0, *t_n0, *dp_n0, *v_nm1, *t_nm1, *dp_nm1;
+  const double *qdp_n0;  // moist: tracer block of the n0_qdp level (tracer 0 used), else nullptr
+  double *vn0, *omega, *phi;
+  double *out_v, *out_t, *out_dp;
+  double dt, eta_ave_w, p_top;  // p_top = hyai(0)*ps0
+};
+
+constexpr int DSS_MAX_FIELDS = 8;
+struct DssField {
+  const double *in;
+  double *out;
+  long long estride;  // elements
+  int lstride;        // levels
+};
+struct DssArgs {
+  DssField f[DSS_MAX_FIELDS];
+  int nf, nelem, nlev;
+  const int *src;        // [ie][16][3] global point ids (ie*16 + pt) or -1
+  const double *rsmp;    // multiply by rspheremp if non-null
+};
+
+struct HvArgs {
+  MetricPtrs m;
+  const int *src;
+  int nelem, nlev;
+  double *v, *t, *dp;                // state at np1 (in/out)
+  double *vt_a, *tt_a, *dpt_a;      // pre-exchange buffers
+  double *vt_b, *tt_b, *dpt_b;      // second buffers
+  double *dpdiss_ave, *dpdiss_bih;
+  double dt, eta_ave_w, nu, nu_s, nu_p, nu_top, nu_ratio1, nu_ratio2;
+  int nsub;
+};
+
+struct EulerArgs {
+  MetricPtrs m;
+  const int *src, *nbr;
+  int nelem, nlev, qsize;
+  const double *qdp_n0;       // [ie][q][lev][pt]
+  double *qdp_pre;            // pre-DSS output
+  double *qdp_np1;            // final output (DSS)
+  double *qbih, *qbih_pre;    // biharmonic buffers
+  double *qlim, *qlim_new;    // [ie][q][2][lev]
+  const double *derived_dp, *divdp, *divdp_proj, *vn0, *dpdiss_ave, *dpdiss_bih, *dp0;
+  double *divdp_w, *divdp_proj_w;
+  double *fdss, *fdss_pre;    // the extra DSS field and its pre-exchange copy
+  double dt, rhs_multiplier, rhs_viss, nu_p, nu_q;
+  int limiter_option;
+  const double *qdp_avg_n0;   // stage 3: Qdp(n0_qdp) for the time average, else nullptr
+};
+
+struct RemapArgs {
+  int nelem, nlev, qsize, remap_alg;
+  double *v, *t;              // state at np1
+  const double *dp;           // source thickness dp3d(np1)
+  double *ps;                 // ps_v(np1) [ie][pt]
+  double *qdp;                // tracers at np1_qdp
+  double *Q;                  // tracer mixing ratio out (update_q)
+  const double *hyai, *hybi;  // nlev+1
+  double ps0, hyai0;
+  int *error;                 // set to 1 on invalid thickness, 2 on unsupported deformation
+};
+
+// launchers (each enqueues on `st`)
+void launch_dp3d_from_ps(hipStream_t st, int nelem, int nlev, const double *hyai_d, const double *hybi_d, double ps0,
+                         const double *ps, double *dp);
+void launch_prim_step_init(hipStream_t st, int nelem, int nlev, bool zero_diss, const double *dp_n0,
+                           double *eta_dot, double *vn0, double *omega, double *dpa, double *dpb, double *derived_dp);
+void launch_caar(hipStream_t st, const CaarArgs &a);
+void launch_rk_combo(hipStream_t st, int nelem, int nlev, double *v_nm1, double *t_nm1, double *dp_nm1,
+                     const double *v_n0, const double *t_n0, const double *dp_n0);
+void launch_dss(hipStream_t st, const DssArgs &a);
+void launch_hv_cycle(hipStream_t st, const HvArgs &a);
+void launch_euler_divdp(hipStream_t st, const EulerArgs &a);
+void launch_euler_qminmax(hipStream_t st, const EulerArgs &a, int write_q);
+void launch_euler_nbr_minmax(hipStream_t st, const EulerArgs &a);
+void launch_euler_biharmonic(hipStream_t st, const EulerArgs &a);
+void launch_euler_advect(hipStream_t st, const EulerArgs &a);
+void launch_euler_dss(hipStream_t st, const EulerArgs &a, int avg);
+void launch_state_forcing(hipStream_t st, int nelem, int nlev, double dt, const double *ft, const double *fm,
+                          double *t, double *v);
+void launch_tracer_forcing(hipStream_t st, int nelem, int nlev, int qsize, bool moist, double dt, const double *fq,
+                           double *qdp, double *Q, const double *hyai, const double *hybi, double ps0, double *ps);
+void launch_remap(hipStream_t st, const RemapArgs &a);
+void launch_zero_int(hipStream_t st, int *p);
+
+// XCD-aware block order: consecutive logical blocks stay on one XCD (one L2)
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+}  // namespace hx

@@ -56,6 +56,7 @@ def lib():
         L.oracle_unit_ppm.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]
         L.oracle_unit_dss.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int]
         L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_debug_stop.argtypes = [_vp, ctypes.c_int]
         _lib = L
     return _lib
 

@@ -74,6 +74,7 @@ typedef struct {
   /* time levels */
   int nm1, n0, np1, nstep, nstep0, n0_qdp, np1_qdp;
   int error;
+  int debug_stop, stopped;
 } Oracle;
 
 /* ------------------------------------------------------------------ layout */
@@ -550,8 +551,14 @@ static void caar_run(Oracle *o, int nm1, int n0, int np1, double dt, double eta_
   dss(o, 4, f, es, L, o->rspheremp);
 }
 
+static int stop_here(Oracle *o, int phase) {
+  if (o->debug_stop == phase) o->stopped = 1;
+  return o->stopped;
+}
+
 static void u3_5stage(Oracle *o, int nm1, int n0, int np1, double dt, double eta_ave_w, double *work) {
   caar_run(o, n0, n0, nm1, dt / 5.0, eta_ave_w / 4.0, work);
+  if (stop_here(o, 1)) return;
   caar_run(o, n0, nm1, np1, dt / 5.0, 0.0, work);
   caar_run(o, n0, np1, np1, dt / 3.0, 0.0, work);
   caar_run(o, n0, np1, np1, 2.0 * dt / 3.0, 0.0, work);
@@ -905,6 +912,7 @@ static void advec_tracers_rk2(Oracle *o, double dt, double *work) {
     memcpy(o->divdp_proj + (size_t)ie * E3, o->divdp + (size_t)ie * E3, sizeof(double) * E3);
   }
   euler_step(o, o->np1_qdp, o->n0_qdp, dt / 2.0, 0.0, DSS_DIV_VDP_AVE, &rhs_viss, work);
+  if (stop_here(o, 4)) return;
   euler_step(o, o->np1_qdp, o->np1_qdp, dt / 2.0, 1.0, DSS_ETA, &rhs_viss, work);
   euler_step(o, o->np1_qdp, o->np1_qdp, dt / 2.0, 2.0, DSS_OMEGA, &rhs_viss, work);
   const double rkstage = 3.0;
@@ -1130,7 +1138,9 @@ static void prim_step(Oracle *o, double dt, double *work) {
     }
     const double eta_ave_w = 1.0 / o->qsplit;
     u3_5stage(o, o->nm1, o->n0, o->np1, dt, eta_ave_w, work);
+    if (o->stopped || stop_here(o, 2)) return;
     hv_run(o, o->np1, dt, eta_ave_w, work);
+    if (stop_here(o, 3)) return;
   }
   if (o->qsize > 0) {
     const int it = o->nstep / o->qsplit;
@@ -1138,6 +1148,7 @@ static void prim_step(Oracle *o, double dt, double *work) {
     o->np1_qdp = 1 - o->n0_qdp;
     advec_tracers_rk2(o, dt * o->qsplit, work);
   }
+  stop_here(o, 5);
 }
 
 /* ============================================================ public API */
@@ -1337,7 +1348,13 @@ int oracle_run_subcycle(Oracle *o, double dt, int *nstep, int *nm1, int *n0, int
     for (int p = 0; p < NP2; ++p)
       for (int k = 0; k < L; ++k)
         DP(ie, o->n0)[p * L + k] = o->hyai_delta[k] * o->ps0 + o->hybi_delta[k] * PSV(ie, o->n0)[p];
+  o->stopped = 0;
   prim_step(o, dt, work);
+  if (o->stopped) {
+    free(work);
+    *nstep = o->nstep; *nm1 = o->nm1; *n0 = o->n0; *np1 = o->np1;
+    return 0;
+  }
   for (int r = 1; r < o->rsplit; ++r) {
     const int tmp = o->np1;
     o->np1 = o->nm1; o->nm1 = o->n0; o->n0 = tmp; ++o->nstep;
@@ -1463,3 +1480,5 @@ void oracle_unit_dss(Oracle *o, int nf, int nl, double *fields_f90, int apply_rs
 }
 
 int oracle_num_threads(void) { return nthreads(); }
+
+void oracle_debug_stop(Oracle *o, int phase) { o->debug_stop = phase; }
