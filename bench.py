@@ -1,0 +1,203 @@
+"""Benchmark of the preqx step on one MI355X (FP64).
+
+Workload: ne30 (5400 elements), np4, L72, 40 tracers, JW baroclinic jet with
+perturbation, dt = 300 s, hyperviscosity subcycled 3x (nu = 1e15, nu_top =
+2.5e5), remap every step, limiter 8, PPM remap alg 1, no forcing.  One step =
+one prim_run_subcycle_c call through the C-ABI with all state resident in HBM.
+
+Prints one JSON line: SYPD = dt / (365 * seconds per step), the roofline of the
+dominant kernel (HIP-event durations on the library stream over the timed
+steps; algorithmic bytes from the kernel's array traffic), the whole-step
+fraction of HBM peak under the canonical bytes-per-step model, and a CPU
+baseline (the oracle, OpenMP over elements, one step of the same workload).
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "SYPD @ne30 L72 Q40, 1 GPU: FP64 vs mixed (<=1e-6 rel); adjoint/fwd cost"
+
+
+def f3_bytes(nelem, nlev):
+    return nelem * 16 * nlev * 8
+
+
+def step_model_bytes(nelem, nlev, q, h=3, r=1):
+    """Canonical algorithmic bytes of one dynamics step (multiples of one FP64
+    3-D field F3): prologue 3, RK5 109, HV H*(48 + 12/L), tracers 27.375Q + 57,
+    remap (2Q + 8)/r, update_q 2Q/r."""
+    b = 3 + 109 + h * (48 + 12.0 / nlev) + (27.375 * q + 57) + (2 * q + 8) / r + 2 * q / r
+    return b * f3_bytes(nelem, nlev)
+
+
+def kernel_bytes(name, nelem, nlev, q, launches):
+    """Algorithmic bytes of one launch of a timed phase: every array the phase
+    must read once plus every array it writes once (F3 = one FP64 3-D field,
+    qlim = q/8 F3; element metric terms are < 0.1 F3 and left out).  Phases
+    with several launches per step return the average over one step's
+    launches."""
+    F = f3_bytes(nelem, nlev)
+    ql = q / 8.0
+    if name == "euler_advect":
+        # reads Qdp(Q), derived_dp, divdp_proj, vn0(2), divdp, fdss, qlim; writes qdp_pre(Q), fdss_pre
+        # stage 2 also writes qlim; stage 3 also reads qbih(Q) and dpdiss_bih
+        s1 = (q + 6 + ql) + (q + 1)
+        s2 = s1 + ql
+        s3 = s1 + q + 1
+        return (s1 + s2 + s3) / 3.0 * F
+    if name == "euler_dss":
+        # all: qdp_pre(Q) + fdss_pre in, Qdp(Q) + fdss out
+        # stage 2: + derived_dp, divdp_proj, dpdiss_ave in, qbih_pre(Q), qlim out; stage 3: + Qdp(n0)(Q) in
+        s1 = 2 * q + 2
+        return (s1 + (s1 + 3 + q + ql) + (s1 + q)) / 3.0 * F
+    if name == "euler_bih_post":
+        return 2 * q * F
+    if name == "euler_divdp_qmm":
+        return (q + 3 + 2 + ql) * F
+    if name == "caar":
+        return (10 + 3 + 1 + 4) * F  # reads n0(4), nm1(4), vn0(2), omega; writes 4, vn0(2), omega, phi
+    if name == "caar_dss":
+        return 8 * F
+    if name == "remap":
+        return ((3 + q) * 2 + 1 + q) * F  # v,T,Qdp in/out, dp3d, Q out
+    return None
+
+
+def progress(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def run_gpu(args):
+    import numpy as np
+    from hommexx_amd.hip import HipSession
+    from hommexx_amd.problem import build_problem
+
+    prob = build_problem(ne=args.ne, nlev=args.nlev, qsize=args.qsize, seed=1, dt=args.dt, nu=1e15, nu_p=1e15,
+                         nu_q=1e15, nu_s=1e15, nu_div=1e15, nu_top=2.5e5, hypervis_subcycle=3, rsplit=1)
+    progress("problem built: nelem %d nlev %d qsize %d" % (prob.nelem, prob.nlev, prob.qsize))
+    sph = prob.metric["spheremp"]
+    with HipSession(prob, abort_on_error=True) as h:
+        progress("state resident on the GPU")
+        h.run(args.warmup)
+        h.synchronize()
+        progress("%d warmup steps done" % args.warmup)
+        h.profile(True)
+        h.profile_reset()
+        t0 = time.perf_counter()
+        h.run(args.steps)
+        h.synchronize()
+        t1 = time.perf_counter()
+        progress("%d timed steps done: %.3f ms/step" % (args.steps, (t1 - t0) / args.steps * 1e3))
+        h.profile(False)
+        prof = h.profile_report()
+        st = h.state()
+        nstep, nm1, n0, np1 = h.tl
+    sec = (t1 - t0) / args.steps
+    # conservation check over warmup + timed steps
+    dry0 = float((sph * prob.state["ps_v"][:, 1]).sum())
+    dry = float((sph * st["ps_v"][:, n0]).sum())
+    q0 = np.einsum("epq,eclpq->c", sph, prob.state["Qdp"][:, 0, :args.qsize])
+    q1 = np.einsum("epq,eclpq->c", sph, st["Qdp"][:, nstep % 2, :args.qsize])
+    return prob, sec, prof, abs(dry / dry0 - 1.0), float(np.max(np.abs(q1 / q0 - 1.0)))
+
+
+def run_cpu(prob, min_seconds=10.0, max_steps=20):
+    """The oracle on the host cores: steps of the same workload until about
+    ``min_seconds`` of CPU work (at least one step)."""
+    import oracle
+    oracle.build()
+    s = oracle.OracleSession(prob)
+    t0 = time.perf_counter()
+    s.run(1)
+    n = 1
+    while time.perf_counter() - t0 < min_seconds and n < max_steps:
+        s.run(1)
+        n += 1
+    t1 = time.perf_counter()
+    progress("cpu baseline: %d steps in %.2f s" % (n, t1 - t0))
+    threads = oracle.lib().oracle_num_threads()
+    s.close()
+    return (t1 - t0) / n, n, threads
+
+
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """A plain device copy of known size, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters: each launch reads and writes ``nbytes``."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ne", type=int, default=30)
+    ap.add_argument("--nlev", type=int, default=72)
+    ap.add_argument("--qsize", type=int, default=40)
+    ap.add_argument("--dt", type=float, default=300.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the preqx step here is built and measured on a single MI355X; "
+              "multi-GPU runs are not part of this build (see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    prob, sec, prof, dry_drift, q_drift = run_gpu(args)
+    nelem, nlev, q = prob.nelem, prob.nlev, args.qsize
+    sypd = args.dt / (365.0 * sec)
+    # dominant kernel by total time
+    kern = {k: v for k, v in prof.items() if kernel_bytes(k, nelem, nlev, q, 1) is not None}
+    dom = max(kern, key=lambda k: kern[k][1]) if kern else None
+    roof = None
+    if dom:
+        cnt, tot = kern[dom]
+        avg_s = tot / cnt / 1e3
+        byts = kernel_bytes(dom, nelem, nlev, q, cnt)
+        ach = byts / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": byts,
+                "avg_launch_ms": round(avg_s * 1e3, 4), "launches": cnt}
+    model = step_model_bytes(nelem, nlev, q)
+    step_roof = {"model_bytes_per_step": model, "achieved": round(model / sec / 1e9, 1),
+                 "frac": round(model / sec / 1e9 / HBM_PEAK_GBS, 4)}
+    phases = {k: {"launches": v[0], "ms_per_step": round(v[1] / args.steps, 4)} for k, v in sorted(prof.items())}
+    cpu = None
+    if not args.no_cpu:
+        csec, nsteps, threads = run_cpu(prob)
+        cpu = {"value": round(args.dt / (365.0 * csec), 6), "unit": "SYPD", "cores": threads, "kind": "port",
+               "sample": "%d prim_run_subcycle_c steps of the same ne%d L%d Q%d workload on the oracle "
+                         "(C, OpenMP over elements): %.3f s/step" % (nsteps, args.ne, nlev, q, csec)}
+    out = {
+        "metric": METRIC, "value": round(sypd, 3), "unit": "SYPD", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "preqx step ne%d np4 L%d Q%d, dt=%gs, hv subcycle 3, remap every step, limiter 8, "
+                               "JW baroclinic" % (args.ne, nlev, q, args.dt),
+                   "nelem": nelem, "nlev": nlev, "qsize": q, "form": "fp64"},
+        "roofline": roof, "step_roofline": step_roof, "cpu_baseline": cpu,
+        "conservation": {"dry_mass_rel_drift": dry_drift, "tracer_mass_rel_drift_max": q_drift},
+        "phases": phases,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

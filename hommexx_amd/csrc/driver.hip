@@ -440,51 +440,54 @@ void advec_tracers(double dt) {
   a.nu_q = c.p.nu_q;
   a.limiter_option = c.p.limiter_option;
   a.rhs_viss = 0.0;
-  {
-    Scope s("euler_divdp");
-    launch_euler_divdp(c.st, a);
-  }
-  int cur = 0;
   const double dts = dt / 2.0;
+  a.dt = dts;
+  int cur = 0;
+  // stage 1 bounds: divdp, then qmin/qmax of Qdp(n0_qdp) (rhs_multiplier 0)
+  a.rhs_multiplier = 0.0;
+  a.qdp_n0 = c.qdp[c.n0_qdp];
+  a.qlim = c.qlim[cur];
+  {
+    Scope s("euler_divdp_qmm");
+    launch_euler_divdp_qmm(c.st, a);
+  }
+  auto nbr_minmax = [&]() {
+    a.qlim = c.qlim[cur];
+    a.qlim_new = c.qlim[1 - cur];
+    Scope s("euler_nbr_minmax");
+    launch_euler_nbr_minmax(c.st, a);
+    cur = 1 - cur;
+    a.qlim = c.qlim[cur];
+  };
+  nbr_minmax();
   for (int stage = 0; stage < 3; ++stage) {
     const int n0q = stage == 0 ? c.n0_qdp : c.np1_qdp;
     a.qdp_n0 = c.qdp[n0q];
     a.qdp_np1 = c.qdp[c.np1_qdp];
-    a.dt = dts;
     a.rhs_multiplier = (double)stage;
-    a.qlim = c.qlim[cur];
-    a.qlim_new = c.qlim[1 - cur];
-    {
-      Scope s("euler_qminmax");
-      launch_euler_qminmax(c.st, a, stage == 2);
-    }
-    if (stage == 0) {
-      Scope s("euler_nbr_minmax");
-      launch_euler_nbr_minmax(c.st, a);
-      cur = 1 - cur;
-    } else if (stage == 2) {
-      a.rhs_viss = 3.0;
-      {
-        Scope s("euler_biharmonic");
-        launch_euler_biharmonic(c.st, a);
-      }
-      Scope s("euler_nbr_minmax");
-      launch_euler_nbr_minmax(c.st, a);
-      cur = 1 - cur;
-    }
-    a.qlim = c.qlim[cur];
-    a.qlim_new = c.qlim[1 - cur];
+    a.rhs_viss = stage == 2 ? 3.0 : 0.0;
     a.fdss = stage == 0 ? c.divdp_proj : stage == 1 ? c.eta_dot : c.omega;
     a.qdp_avg_n0 = stage == 2 ? c.qdp[c.n0_qdp] : nullptr;
     {
       Scope s("euler_advect");
-      launch_euler_advect(c.st, a);
+      launch_euler_advect(c.st, a, stage + 1);
     }
     {
+      // stage 2's exchange also computes the stage-3 bounds and the first
+      // laplacian of the tracer biharmonic
       Scope s("euler_dss");
-      launch_euler_dss(c.st, a, stage == 2);
+      launch_euler_dss(c.st, a, stage == 0 ? 0 : stage == 1 ? 1 : 2, dts);
     }
     if (stage == 0 && stop_here(4)) return;
+    if (stage == 1) {
+      EulerArgs b = a;
+      b.rhs_viss = 3.0;
+      {
+        Scope s("euler_bih_post");
+        launch_euler_bih_post(c.st, b);
+      }
+      nbr_minmax();
+    }
   }
 }
 

@@ -1,12 +1,27 @@
 // Euler tracer step with quasi-monotone limiters (EulerStepFunctorImpl.hpp
-// in the reference).  Slab kernels (16 lanes = one element at one level).
-// The advection kernel keeps one slab per 16 lanes and loops over all
-// tracers, so the per-level inputs (vstar, dpdissk, metric) are loaded once
-// and reused qsize times.  Limiter sums are serial over the 16 points
-// (slab_sum_serial) to round like the reference.
+// and prim_advec_tracers_remap.cpp in the reference).
+//
+// Slab kernels: 16 lanes = one element at one level.  Kernels without
+// neighbour reads loop over all tracers of a slab (metric, dp and vstar are
+// loaded once per slab); kernels that gather neighbour copies (the DSS) loop
+// over a chunk of tracers so the neighbours' lines are still in L2 when they
+// are read.  One remap interval's tracer step is
+//   divdp_qmm   : divdp = div(vn0); stage-1 qmin/qmax
+//   nbr_minmax
+//   advect(1)   ; dss(1)  (extra field divdp_proj)
+//   advect(2)   (stage-2 qmin/qmax update fused: no exchange at this stage)
+//   dss(2)      (extra field eta_dot_dpdn; fused: stage-3 qmin/qmax and the
+//                first laplacian of the tracer biharmonic)
+//   bih_post    (DSS of the first laplacian, second laplacian, scaling)
+//   nbr_minmax
+//   advect(3)   ; dss(3) (extra field omega_p; fused SSP average)
+// Limiter sums run over the 16 points in index order (slab_sum_serial) to
+// round like the reference's serial reductions.
 #include "kernels.hpp"
 
 namespace hx {
+
+constexpr int QCHUNK = 8;
 
 __device__ __forceinline__ bool slab_ie_k(size_t nslab_total, int nlev, size_t &slab, int &ie, int &k, int &pt) {
   const int b = xcd_block(blockIdx.x, gridDim.x);
@@ -18,51 +33,67 @@ __device__ __forceinline__ bool slab_ie_k(size_t nslab_total, int nlev, size_t &
   return true;
 }
 
-// divdp = div(vn0), divdp_proj = divdp (EulerStepFunctorImpl.hpp:348-377)
-__global__ __launch_bounds__(256) void euler_divdp_kernel(EulerArgs a) {
+// (tracer chunk, slab) decomposition with the chunk slowest
+__device__ __forceinline__ bool chunk_slab(const EulerArgs &a, int &q0, int &q1, size_t &slab, int &ie, int &k,
+                                           int &pt) {
+  const int nb = (int)(((size_t)a.nelem * a.nlev + 15) / 16);
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int chunk = lb / nb, sb = lb % nb;
+  q0 = chunk * QCHUNK;
+  q1 = q0 + QCHUNK < a.qsize ? q0 + QCHUNK : a.qsize;
+  slab = (size_t)sb * 16 + (threadIdx.x >> 4);
+  pt = threadIdx.x & 15;
+  if (slab >= (size_t)a.nelem * a.nlev) return false;
+  ie = (int)(slab / a.nlev);
+  k = (int)(slab % a.nlev);
+  return true;
+}
+
+// metric needed by divergence / weak laplacian only
+__device__ __forceinline__ void load_elem_light(const MetricPtrs &m, int ie, int pt, ElemLane &e) {
+  const size_t t = (size_t)ie * 64 + pt, s = (size_t)ie * 16 + pt;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) e.Dinv[j] = m.Dinv[t + 16 * j];
+  e.metdet = m.metdet[s];
+  e.sph = m.spheremp[s];
+  e.rsmp = m.rspheremp[s];
+}
+
+__device__ __forceinline__ double gather(const double *in, size_t es, size_t off, int ie, int pt, int s0, int s1,
+                                         int s2) {
+  double x = in[(size_t)ie * es + off + pt];
+  if (s0 >= 0) x += in[(size_t)(s0 >> 4) * es + off + (s0 & 15)];
+  if (s1 >= 0) x += in[(size_t)(s1 >> 4) * es + off + (s1 & 15)];
+  if (s2 >= 0) x += in[(size_t)(s2 >> 4) * es + off + (s2 & 15)];
+  return x;
+}
+
+// --------------------------------------------------------------------------
+// divdp = div(vn0), divdp_proj = divdp (EulerStepFunctorImpl.hpp:348-377) and
+// the stage-1 compute_dp + compute_qmin_qmax (:406-485)
+__global__ __launch_bounds__(256) void euler_divdp_qmm_kernel(EulerArgs a) {
   size_t slab;
   int ie, k, pt;
   if (!slab_ie_k((size_t)a.nelem * a.nlev, a.nlev, slab, ie, k, pt)) return;
+  const int L = a.nlev, Q = a.qsize;
   ElemLane e;
-  load_elem(a.m, ie, pt, e);
+  load_elem_light(a.m, ie, pt, e);
   DvvLane d;
   load_dvv(a.m.dvv, pt, d);
   const size_t s = slab * 16 + pt, v = slab * 32 + pt;
   const double dv = sph_div(e, d, a.vn0[v], a.vn0[v + 16]);
   a.divdp_w[s] = dv;
   a.divdp_proj_w[s] = dv;
-}
-
-void launch_euler_divdp(hipStream_t st, const EulerArgs &a) {
-  const size_t nslab = (size_t)a.nelem * a.nlev;
-  hipLaunchKernelGGL(euler_divdp_kernel, dim3((nslab + 15) / 16), dim3(256), 0, st, a);
-}
-
-// compute_dp + compute_qmin_qmax (:406-485).  Slab = (ie, q, lev).
-__global__ __launch_bounds__(256) void euler_qminmax_kernel(EulerArgs a, int write_q) {
-  const int b = xcd_block(blockIdx.x, gridDim.x);
-  const size_t slab = (size_t)b * 16 + (threadIdx.x >> 4);
-  const int pt = threadIdx.x & 15;
-  const int L = a.nlev, Q = a.qsize;
-  if (slab >= (size_t)a.nelem * Q * L) return;
-  const int k = (int)(slab % L);
-  const int q = (int)((slab / L) % Q);
-  const int ie = (int)(slab / ((size_t)L * Q));
-  const size_t s = ((size_t)ie * L + k) * 16 + pt;
   const double rhsmdt = a.rhs_multiplier * a.dt;
-  const double dp = a.derived_dp[s] - rhsmdt * a.divdp_proj[s];
-  const size_t qi = slab * 16 + pt;  // [ie][q][lev][pt]
-  const double v = a.qdp_n0[qi] / dp;
-  if (write_q) a.qbih[qi] = v;
-  const double mn = slab_min(v), mx = slab_max(v);
-  if (pt == 0) {
-    double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
-    if (a.rhs_multiplier != 1.0) {
+  const double dp = a.derived_dp[s] - rhsmdt * dv;
+  for (int q = 0; q < Q; ++q) {
+    const size_t qi = (((size_t)ie * Q + q) * L + k) * 16 + pt;
+    const double x = a.qdp_n0[qi] / dp;
+    const double mn = slab_min(x), mx = slab_max(x);
+    if (pt == 0) {
+      double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
       ql[k] = mn;
       ql[L + k] = mx;
-    } else {
-      ql[k] = dmin(ql[k], mn);
-      ql[L + k] = dmax(ql[L + k], mx);
     }
   }
 }
@@ -86,78 +117,6 @@ __global__ __launch_bounds__(256) void euler_nbr_minmax_kernel(EulerArgs a) {
   }
   a.qlim_new[own] = mn;
   a.qlim_new[own + L] = mx;
-}
-
-// biharmonic pre (:196-267): qbih <- laplace(qbih * dpdiss_ave / dp0)
-__global__ __launch_bounds__(256) void euler_bih_pre_kernel(EulerArgs a) {
-  const int b = xcd_block(blockIdx.x, gridDim.x);
-  const size_t slab = (size_t)b * 16 + (threadIdx.x >> 4);
-  const int pt = threadIdx.x & 15;
-  const int L = a.nlev, Q = a.qsize;
-  if (slab >= (size_t)a.nelem * Q * L) return;
-  const int k = (int)(slab % L);
-  const int ie = (int)(slab / ((size_t)L * Q));
-  ElemLane e;
-  load_elem(a.m, ie, pt, e);
-  DvvLane d;
-  load_dvv(a.m.dvv, pt, d);
-  const size_t qi = slab * 16 + pt;
-  double x = a.qbih[qi];
-  if (a.nu_p > 0) x = x * a.dpdiss_ave[((size_t)ie * L + k) * 16 + pt] / a.dp0[k];
-  a.qbih_pre[qi] = sph_laplace(e, d, x);
-}
-
-// biharmonic post (:216-310): laplace of the DSS'd field, times -rhs_viss*dt*nu_q*dp0/spheremp
-__global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
-  const int b = xcd_block(blockIdx.x, gridDim.x);
-  const size_t slab = (size_t)b * 16 + (threadIdx.x >> 4);
-  const int pt = threadIdx.x & 15;
-  const int L = a.nlev, Q = a.qsize;
-  if (slab >= (size_t)a.nelem * Q * L) return;
-  const int k = (int)(slab % L);
-  const int ie = (int)(slab / ((size_t)L * Q));
-  ElemLane e;
-  load_elem(a.m, ie, pt, e);
-  DvvLane d;
-  load_dvv(a.m.dvv, pt, d);
-  const size_t qi = slab * 16 + pt;
-  const double lap = sph_laplace(e, d, a.qbih[qi]);
-  const double f = -a.rhs_viss * a.dt * a.nu_q;
-  a.qbih[qi] = (f * a.dp0[k] * lap / e.sph);
-}
-
-// DSS of all tracers of an (element, level) slab plus the extra field;
-// at stage 3 the SSP average (Qdp(n0) + 2 Qdp(np1)) / 3 is applied to the
-// assembled value (EulerStepFunctorImpl.hpp:379-403).
-__global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, const double *in, double *out, int with_fdss,
-                                                        int avg) {
-  size_t slab;
-  int ie, k, pt;
-  if (!slab_ie_k((size_t)a.nelem * a.nlev, a.nlev, slab, ie, k, pt)) return;
-  const int L = a.nlev, Q = a.qsize;
-  const int *src = a.src + ((size_t)ie * 16 + pt) * 3;
-  const int s0 = src[0], s1 = src[1], s2 = src[2];
-  const double r = a.m.rspheremp[(size_t)ie * 16 + pt];
-  const size_t es = (size_t)Q * L * 16;
-  for (int q = 0; q < Q; ++q) {
-    const size_t off = ((size_t)q * L + k) * 16;
-    double x = in[(size_t)ie * es + off + pt];
-    if (s0 >= 0) x += in[(size_t)(s0 >> 4) * es + off + (s0 & 15)];
-    if (s1 >= 0) x += in[(size_t)(s1 >> 4) * es + off + (s1 & 15)];
-    if (s2 >= 0) x += in[(size_t)(s2 >> 4) * es + off + (s2 & 15)];
-    x *= r;
-    if (avg) x = (a.qdp_avg_n0[(size_t)ie * es + off + pt] + (3.0 - 1) * x) / 3.0;
-    out[(size_t)ie * es + off + pt] = x;
-  }
-  if (with_fdss) {
-    const size_t e3 = (size_t)L * 16, off = (size_t)k * 16;
-    double x = a.fdss_pre[(size_t)ie * e3 + off + pt];
-    if (s0 >= 0) x += a.fdss_pre[(size_t)(s0 >> 4) * e3 + off + (s0 & 15)];
-    if (s1 >= 0) x += a.fdss_pre[(size_t)(s1 >> 4) * e3 + off + (s1 & 15)];
-    if (s2 >= 0) x += a.fdss_pre[(size_t)(s2 >> 4) * e3 + off + (s2 & 15)];
-    x *= r;
-    a.fdss[(size_t)ie * e3 + off + pt] = x;
-  }
 }
 
 // ------------------------------------------------------------ limiters
@@ -214,19 +173,19 @@ __device__ __forceinline__ void limit9_slab(double minp, double maxp, double &x,
   }
 }
 
-// Setup phase (:585-626) + tracer phase (:571-687): per (element, level)
-// slab, loop over all tracers.
-__global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a) {
+// Setup phase (:585-626) + tracer phase (:571-687) for every tracer of an
+// (element, level) slab.  At stage 2 the qmin/qmax update of that stage
+// (min/max with the previous bounds, no neighbour exchange) runs first.
+__global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stage) {
   size_t slab;
   int ie, k, pt;
   if (!slab_ie_k((size_t)a.nelem * a.nlev, a.nlev, slab, ie, k, pt)) return;
   const int L = a.nlev, Q = a.qsize;
   ElemLane e;
-  load_elem(a.m, ie, pt, e);
+  load_elem_light(a.m, ie, pt, e);
   DvvLane d;
   load_dvv(a.m.dvv, pt, d);
   const size_t s = slab * 16 + pt, v = slab * 32 + pt;
-  // setup
   const double rhsmdt = a.rhs_multiplier * a.dt;
   const double dp = a.derived_dp[s] - rhsmdt * a.divdp_proj[s];
   const double vs0 = a.vn0[v] / dp, vs1 = a.vn0[v + 16] / dp;
@@ -238,20 +197,26 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a) {
   }
   a.fdss_pre[s] = a.fdss[s] * e.sph;
   const bool add_hv = a.rhs_viss != 0.0;
+  const bool limit = a.limiter_option == 8 || a.limiter_option == 9;
   const double c = e.sph * dpdissk;
+  const double s1 = limit ? slab_sum_serial(c) : 0.0;  // same for every tracer
   for (int q = 0; q < Q; ++q) {
     const size_t qi = (((size_t)ie * Q + q) * L + k) * 16 + pt;
     const double qd = a.qdp_n0[qi];
+    double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
+    double minp = ql[k], maxp = ql[L + k];
+    if (stage == 2) {  // compute_qmin_qmax with rhs_multiplier == 1
+      const double xq = qd / dp;
+      minp = dmin(minp, slab_min(xq));
+      maxp = dmax(maxp, slab_max(xq));
+    }
     const double q0 = add_hv ? a.qbih[qi] : 0;
     double ptens = sph_div_update(e, d, -a.dt, vs0, vs1, qd, q0);
-    if (a.limiter_option == 8 || a.limiter_option == 9) {
+    bool wmin = stage == 2, wmax = stage == 2;
+    if (limit) {
       double x = ptens / dpdissk;
-      double s0, s1;
-      slab_sum_serial2(x * c, c, s0, s1);
+      const double s0 = slab_sum_serial(x * c);
       if (!(s1 <= 0)) {
-        double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
-        double minp = ql[k], maxp = ql[L + k];
-        bool wmin = false, wmax = false;
         if (minp < 0) {
           minp = 0;
           wmin = true;
@@ -264,22 +229,102 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a) {
           maxp = s0 / s1;
           wmax = true;
         }
-        if (pt == 0) {
-          if (wmin) ql[k] = minp;
-          if (wmax) ql[L + k] = maxp;
-        }
         if (a.limiter_option == 8) limit8_slab(s0, minp, maxp, x, c);
         else limit9_slab(minp, maxp, x, c);
         ptens = x * dpdissk;
       }
     }
+    if (pt == 0) {
+      if (wmin) ql[k] = minp;
+      if (wmax) ql[L + k] = maxp;
+    }
     a.qdp_pre[qi] = e.sph * ptens;
   }
 }
 
-void launch_euler_qminmax(hipStream_t st, const EulerArgs &a, int write_q) {
-  const size_t nq = (size_t)a.nelem * a.qsize * a.nlev;
-  hipLaunchKernelGGL(euler_qminmax_kernel, dim3((nq + 15) / 16), dim3(256), 0, st, a, write_q);
+// DSS of a chunk of tracers plus (chunk 0) the extra field.
+//   mode 0: plain
+//   mode 1: stage 2: also stage-3 compute_dp/qmin_qmax and the first
+//           laplacian of the tracer biharmonic (:196-267) -> qbih_pre
+//   mode 2: stage 3: SSP average (Qdp(n0) + 2 Qdp(np1)) / 3 (:379-403)
+__global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int mode, double dt3) {
+  int q0, q1, ie, k, pt;
+  size_t slab;
+  if (!chunk_slab(a, q0, q1, slab, ie, k, pt)) return;
+  const int L = a.nlev, Q = a.qsize;
+  const int *src = a.src + ((size_t)ie * 16 + pt) * 3;
+  const int s0 = src[0], s1 = src[1], s2 = src[2];
+  const double r = a.m.rspheremp[(size_t)ie * 16 + pt];
+  const size_t es = (size_t)Q * L * 16;
+  const size_t e3 = (size_t)L * 16, so = (size_t)k * 16;
+  ElemLane e;
+  DvvLane d;
+  double dp3 = 0, wdp = 0;
+  if (mode == 1) {
+    load_elem_light(a.m, ie, pt, e);
+    load_dvv(a.m.dvv, pt, d);
+    // stage 3: rhs_multiplier 2, divdp_proj final after stage 1
+    dp3 = a.derived_dp[(size_t)ie * e3 + so + pt] - (2.0 * dt3) * a.divdp_proj[(size_t)ie * e3 + so + pt];
+    wdp = a.dpdiss_ave[(size_t)ie * e3 + so + pt];
+  }
+  for (int q = q0; q < q1; ++q) {
+    const size_t off = ((size_t)q * L + k) * 16;
+    double x = gather(a.qdp_pre, es, off, ie, pt, s0, s1, s2);
+    x *= r;
+    if (mode == 2) x = (a.qdp_avg_n0[(size_t)ie * es + off + pt] + (3.0 - 1) * x) / 3.0;
+    a.qdp_np1[(size_t)ie * es + off + pt] = x;
+    if (mode == 1) {
+      const double qv = x / dp3;
+      const double mn = slab_min(qv), mx = slab_max(qv);
+      if (pt == 0) {
+        double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
+        ql[k] = mn;
+        ql[L + k] = mx;
+      }
+      double y = qv;
+      if (a.nu_p > 0) y = y * wdp / a.dp0[k];
+      a.qbih_pre[(size_t)ie * es + off + pt] = sph_laplace(e, d, y);
+    }
+  }
+  if (q0 == 0) {
+    double x = gather(a.fdss_pre, e3, so, ie, pt, s0, s1, s2);
+    x *= r;
+    a.fdss[(size_t)ie * e3 + so + pt] = x;
+  }
+}
+
+// biharmonic post (:216-310): DSS of the first laplacian (with rspheremp),
+// second laplacian, times -rhs_viss*dt*nu_q*dp0/spheremp
+__global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
+  int q0, q1, ie, k, pt;
+  size_t slab;
+  if (!chunk_slab(a, q0, q1, slab, ie, k, pt)) return;
+  const int L = a.nlev, Q = a.qsize;
+  ElemLane e;
+  load_elem_light(a.m, ie, pt, e);
+  DvvLane d;
+  load_dvv(a.m.dvv, pt, d);
+  const int *src = a.src + ((size_t)ie * 16 + pt) * 3;
+  const int s0 = src[0], s1 = src[1], s2 = src[2];
+  const size_t es = (size_t)Q * L * 16;
+  const double f = -a.rhs_viss * a.dt * a.nu_q;
+  for (int q = q0; q < q1; ++q) {
+    const size_t off = ((size_t)q * L + k) * 16;
+    double x = gather(a.qbih_pre, es, off, ie, pt, s0, s1, s2);
+    x *= e.rsmp;
+    const double lap = sph_laplace(e, d, x);
+    a.qbih[(size_t)ie * es + off + pt] = (f * a.dp0[k] * lap / e.sph);
+  }
+}
+
+static int slab_blocks(const EulerArgs &a) { return (int)(((size_t)a.nelem * a.nlev + 15) / 16); }
+static int chunk_blocks(const EulerArgs &a) {
+  const int nch = (a.qsize + QCHUNK - 1) / QCHUNK;
+  return slab_blocks(a) * (nch > 0 ? nch : 1);
+}
+
+void launch_euler_divdp_qmm(hipStream_t st, const EulerArgs &a) {
+  hipLaunchKernelGGL(euler_divdp_qmm_kernel, dim3(slab_blocks(a)), dim3(256), 0, st, a);
 }
 
 void launch_euler_nbr_minmax(hipStream_t st, const EulerArgs &a) {
@@ -287,24 +332,16 @@ void launch_euler_nbr_minmax(hipStream_t st, const EulerArgs &a) {
   hipLaunchKernelGGL(euler_nbr_minmax_kernel, dim3((n + 255) / 256), dim3(256), 0, st, a);
 }
 
-void launch_euler_biharmonic(hipStream_t st, const EulerArgs &a) {
-  const size_t nq = (size_t)a.nelem * a.qsize * a.nlev;
-  hipLaunchKernelGGL(euler_bih_pre_kernel, dim3((nq + 15) / 16), dim3(256), 0, st, a);
-  const size_t ns = (size_t)a.nelem * a.nlev;
-  hipLaunchKernelGGL(euler_dss_kernel, dim3((ns + 15) / 16), dim3(256), 0, st, a, (const double *)a.qbih_pre, a.qbih,
-                     0, 0);
-  hipLaunchKernelGGL(euler_bih_post_kernel, dim3((nq + 15) / 16), dim3(256), 0, st, a);
+void launch_euler_advect(hipStream_t st, const EulerArgs &a, int stage) {
+  hipLaunchKernelGGL(euler_advect_kernel, dim3(slab_blocks(a)), dim3(256), 0, st, a, stage);
 }
 
-void launch_euler_advect(hipStream_t st, const EulerArgs &a) {
-  const size_t ns = (size_t)a.nelem * a.nlev;
-  hipLaunchKernelGGL(euler_advect_kernel, dim3((ns + 15) / 16), dim3(256), 0, st, a);
+void launch_euler_dss(hipStream_t st, const EulerArgs &a, int mode, double dt3) {
+  hipLaunchKernelGGL(euler_dss_kernel, dim3(chunk_blocks(a)), dim3(256), 0, st, a, mode, dt3);
 }
 
-void launch_euler_dss(hipStream_t st, const EulerArgs &a, int avg) {
-  const size_t ns = (size_t)a.nelem * a.nlev;
-  hipLaunchKernelGGL(euler_dss_kernel, dim3((ns + 15) / 16), dim3(256), 0, st, a, (const double *)a.qdp_pre,
-                     a.qdp_np1, 1, avg);
+void launch_euler_bih_post(hipStream_t st, const EulerArgs &a) {
+  hipLaunchKernelGGL(euler_bih_post_kernel, dim3(chunk_blocks(a)), dim3(256), 0, st, a);
 }
 
 }  // namespace hx

@@ -86,12 +86,11 @@ void launch_rk_combo(hipStream_t st, int nelem, int nlev, double *v_nm1, double 
                      const double *v_n0, const double *t_n0, const double *dp_n0);
 void launch_dss(hipStream_t st, const DssArgs &a);
 void launch_hv_cycle(hipStream_t st, const HvArgs &a);
-void launch_euler_divdp(hipStream_t st, const EulerArgs &a);
-void launch_euler_qminmax(hipStream_t st, const EulerArgs &a, int write_q);
+void launch_euler_divdp_qmm(hipStream_t st, const EulerArgs &a);
 void launch_euler_nbr_minmax(hipStream_t st, const EulerArgs &a);
-void launch_euler_biharmonic(hipStream_t st, const EulerArgs &a);
-void launch_euler_advect(hipStream_t st, const EulerArgs &a);
-void launch_euler_dss(hipStream_t st, const EulerArgs &a, int avg);
+void launch_euler_advect(hipStream_t st, const EulerArgs &a, int stage);
+void launch_euler_dss(hipStream_t st, const EulerArgs &a, int mode, double dt3);
+void launch_euler_bih_post(hipStream_t st, const EulerArgs &a);
 void launch_state_forcing(hipStream_t st, int nelem, int nlev, double dt, const double *ft, const double *fm,
                           double *t, double *v);
 void launch_tracer_forcing(hipStream_t st, int nelem, int nlev, int qsize, bool moist, double dt, const double *fq,

@@ -88,11 +88,16 @@ __device__ __forceinline__ void ppm_parabola(double al, double ar, double a, dou
 __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
   const int nf = 3 + a.qsize;
   const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  // fields fastest: the lanes working on one column run together (dp3d shared in cache)
-  const size_t col = tid / nf;
-  const int f = (int)(tid % nf);
-  if (col >= (size_t)a.nelem * 16) return;
-  const int ie = (int)(col >> 4), pt = (int)(col & 15);
+  // 16 lanes = the 16 columns of one element for one field, so each level is
+  // one coalesced 128-byte access; the fields of an element follow each other
+  // (the element's dp3d columns stay in cache across its fields)
+  const int pt = (int)(tid & 15);
+  const size_t rest = tid >> 4;
+  const int f = (int)(rest % nf);
+  const size_t ie_s = rest / nf;
+  if (ie_s >= (size_t)a.nelem) return;
+  const int ie = (int)ie_s;
+  const size_t col = (size_t)ie * 16 + pt;
   const int L = a.nlev;
   const double *src = a.dp + (size_t)ie * L * 16 + pt;
   double *var;

@@ -57,13 +57,14 @@ class VCoord:
         return da * self.ps0 + db * ps
 
 
-def _smooth_tracers(mesh, nlev, qsize, seed):
+def _smooth_tracers(mesh, nlev, qsize, seed, out=None):
     """Seeded smooth tracers in [0, 1]: a random l<=2 harmonic pattern per
     tracer, scaled into [0.05, 0.95], times a gentle vertical modulation."""
     rng = np.random.default_rng(seed)
     x, y, z = mesh.pos[..., 0], mesh.pos[..., 1], mesh.pos[..., 2]
     basis = [np.ones_like(x), x, y, z, x * y, y * z, x * z, x * x - y * y, 3 * z * z - 1]
-    out = np.zeros((mesh.nelem, qsize, nlev, 4, 4))
+    if out is None:
+        out = np.empty((mesh.nelem, qsize, nlev, 4, 4))
     kk = np.arange(nlev)
     for q in range(qsize):
         c = rng.standard_normal(len(basis))
@@ -123,9 +124,9 @@ def jw_baroclinic(mesh, vc, qsize, seed=0, qsize_d=None, t_noise=0.0, u_perturb=
     dp3d = np.repeat(dp[:, None], 3, axis=1)
     Q = np.zeros((nelem, qsize_d, nlev, 4, 4))
     if qsize > 0:
-        Q[:, :qsize] = _smooth_tracers(mesh, nlev, qsize, seed)
-    Qdp = np.zeros((nelem, 2, qsize_d, nlev, 4, 4))
-    Qdp[:, 0] = Q * dp[:, None]
+        _smooth_tracers(mesh, nlev, qsize, seed, out=Q[:, :qsize])
+    Qdp = np.empty((nelem, 2, qsize_d, nlev, 4, 4))
+    np.multiply(Q, dp[:, None], out=Qdp[:, 0])
     Qdp[:, 1] = Qdp[:, 0]
     return {
         "v": np.ascontiguousarray(v), "T": np.ascontiguousarray(T),

@@ -99,43 +99,43 @@ class CubedSphere:
         dang = 0.5 * math.pi / ne
         nelem = self.nelem
         # per element, per point (j, i): 3-D position and its reference derivatives
+        ei = np.arange(ne)
         pos = np.zeros((nelem, NP, NP, 3))
         dpda = np.zeros((nelem, NP, NP, 3))
         dpdb = np.zeros((nelem, NP, NP, 3))
         corner3d = np.zeros((nelem, 4, 3))  # SW, SE, NW, NE
-        self.face_of = np.zeros(nelem, dtype=np.int64)
+        self.face_of = np.repeat(np.arange(6), ne * ne)
+        # element-local angles: alpha[ei, i], beta[ej, j]
+        al = -0.25 * math.pi + (ei[:, None] + 0.5 * (1.0 + x[None, :])) * dang
+        be = al.copy()
+        ca = np.array([-0.25 * math.pi + ei * dang, -0.25 * math.pi + (ei + 1) * dang]).T  # corners (ei, 2)
         for f, (n, e1, e2) in enumerate(_FACES):
             n = np.array(n, float)
             e1 = np.array(e1, float)
             e2 = np.array(e2, float)
-            for ej in range(ne):
-                for ei in range(ne):
-                    ie = self.elem_index(f, ei, ej)
-                    self.face_of[ie] = f
-                    a0 = -0.25 * math.pi + ei * dang
-                    b0 = -0.25 * math.pi + ej * dang
-                    for j in range(NP):
-                        beta = b0 + 0.5 * (1.0 + x[j]) * dang
-                        tb = math.tan(beta)
-                        for i in range(NP):
-                            alpha = a0 + 0.5 * (1.0 + x[i]) * dang
-                            ta = math.tan(alpha)
-                            X = n + ta * e1 + tb * e2
-                            rn = math.sqrt(X @ X)
-                            r = X / rn
-                            # dr/dalpha = (I - r r^T) sec^2(alpha) e1 / |X|
-                            dXa = e1 / math.cos(alpha) ** 2
-                            dXb = e2 / math.cos(beta) ** 2
-                            dra = (dXa - r * (r @ dXa)) / rn
-                            drb = (dXb - r * (r @ dXb)) / rn
-                            pos[ie, j, i] = r
-                            dpda[ie, j, i] = 0.5 * dang * dra
-                            dpdb[ie, j, i] = 0.5 * dang * drb
-                    for k, (ii, jj) in enumerate(((0, 0), (1, 0), (0, 1), (1, 1))):
-                        aa = a0 + ii * dang
-                        bb = b0 + jj * dang
-                        X = n + math.tan(aa) * e1 + math.tan(bb) * e2
-                        corner3d[ie, k] = X / math.sqrt(X @ X)
+            # grids over (ej, j, ei, i)
+            A = al[None, None, :, :]
+            B = be[:, :, None, None]
+            ta, tb = np.tan(A), np.tan(B)
+            X = n + ta[..., None] * e1 + tb[..., None] * e2
+            rn = np.sqrt(np.sum(X * X, axis=-1))[..., None]
+            r = X / rn
+            dXa = (e1 / np.cos(A)[..., None] ** 2) * np.ones_like(tb)[..., None]
+            dXb = (e2 / np.cos(B)[..., None] ** 2) * np.ones_like(ta)[..., None]
+            dra = (dXa - r * np.sum(r * dXa, axis=-1)[..., None]) / rn
+            drb = (dXb - r * np.sum(r * dXb, axis=-1)[..., None]) / rn
+            sl = slice(f * ne * ne, (f + 1) * ne * ne)
+            # (ej, j, ei, i) -> element index ej*ne + ei, point (j, i)
+            to_elem = lambda a: np.transpose(a, (0, 2, 1, 3, 4)).reshape(ne * ne, NP, NP, 3)
+            pos[sl] = to_elem(r)
+            dpda[sl] = to_elem(0.5 * dang * dra)
+            dpdb[sl] = to_elem(0.5 * dang * drb)
+            for k, (ii, jj) in enumerate(((0, 0), (1, 0), (0, 1), (1, 1))):
+                AA = ca[:, ii][None, :]
+                BB = ca[:, jj][:, None]
+                Xc = n + np.tan(AA)[..., None] * e1 + np.tan(BB)[..., None] * e2
+                Xc = Xc / np.sqrt(np.sum(Xc * Xc, axis=-1))[..., None]
+                corner3d[sl, k] = Xc.reshape(ne * ne, 3)
         # snap near-zero coordinates (the poles and face centres are nodes when ne is even)
         pos[np.abs(pos) < 1e-14] = 0.0
         self.pos = pos

@@ -1464,6 +1464,7 @@ void oracle_unit_ppm(int nlev, int remap_alg, const double *src, const double *t
 
 /* DSS of nf 3-D fields given in Fortran layout [ie][lev][np][np] (each field nelem*nlev*16) */
 void oracle_unit_dss(Oracle *o, int nf, int nl, double *fields_f90, int apply_rspheremp) {
+  if (nf <= 0) return;
   const size_t e3 = (size_t)NP2 * L;
   double *tmp = alloc0((size_t)nf * o->nelem * e3);
   double **f = (double **)malloc(sizeof(double *) * nf);
