@@ -473,15 +473,20 @@ void advec_tracers(double dt) {
       launch_euler_advect(c.st, a, stage + 1);
     }
     {
-      // stage 2's exchange also computes the stage-3 bounds and the first
-      // laplacian of the tracer biharmonic
       Scope s("euler_dss");
-      launch_euler_dss(c.st, a, stage == 0 ? 0 : stage == 1 ? 1 : 2, dts);
+      launch_euler_dss(c.st, a, stage == 2);
     }
     if (stage == 0 && stop_here(4)) return;
     if (stage == 1) {
+      // stage-3 bounds and tracer biharmonic from the assembled stage-2 Qdp
       EulerArgs b = a;
+      b.qdp_n0 = c.qdp[c.np1_qdp];
+      b.rhs_multiplier = 2.0;
       b.rhs_viss = 3.0;
+      {
+        Scope s("euler_qmm_lap");
+        launch_euler_qmm_lap(c.st, b);
+      }
       {
         Scope s("euler_bih_post");
         launch_euler_bih_post(c.st, b);

@@ -10,33 +10,40 @@
 
 namespace hx {
 
+// One lane = one (element, level, point) for every field of the call: the
+// source table is read once and all the fields' gathers are issued before
+// any store, so each lane has up to 4*nf independent loads in flight.
 __global__ __launch_bounds__(256) void dss_kernel(DssArgs a) {
-  const int nblk = gridDim.x;
-  const int b = xcd_block(blockIdx.x, nblk);
-  const size_t slab = (size_t)b * 16 + (threadIdx.x >> 4);
-  const int pt = threadIdx.x & 15;
-  const size_t nslab = (size_t)a.nelem * a.nlev;
-  if (slab >= nslab) return;
-  const int ie = (int)(slab / a.nlev), k = (int)(slab % a.nlev);
+  int ie, k, pt;
+  if (!group_slab(a.nelem, a.nlev, ie, k, pt)) return;
   const int *src = a.src + ((size_t)ie * 16 + pt) * 3;
   const int s0 = src[0], s1 = src[1], s2 = src[2];
   const double r = a.rsmp ? a.rsmp[(size_t)ie * 16 + pt] : 1.0;
-  for (int f = 0; f < a.nf; ++f) {
-    const DssField &F = a.f[f];
-    const size_t off = (size_t)k * F.lstride;
-    double x = F.in[(size_t)ie * F.estride + off + pt];
-    if (s0 >= 0) x += F.in[(size_t)(s0 >> 4) * F.estride + off + (s0 & 15)];
-    if (s1 >= 0) x += F.in[(size_t)(s1 >> 4) * F.estride + off + (s1 & 15)];
-    if (s2 >= 0) x += F.in[(size_t)(s2 >> 4) * F.estride + off + (s2 & 15)];
-    if (a.rsmp) x *= r;
-    F.out[(size_t)ie * F.estride + off + pt] = x;
+  double x[DSS_MAX_FIELDS];
+#pragma unroll
+  for (int f = 0; f < DSS_MAX_FIELDS; ++f) {
+    if (f < a.nf) {
+      const DssField &F = a.f[f];
+      const size_t off = (size_t)k * F.lstride;
+      double y = F.in[(size_t)ie * F.estride + off + pt];
+      if (s0 >= 0) y += F.in[(size_t)(s0 >> 4) * F.estride + off + (s0 & 15)];
+      if (s1 >= 0) y += F.in[(size_t)(s1 >> 4) * F.estride + off + (s1 & 15)];
+      if (s2 >= 0) y += F.in[(size_t)(s2 >> 4) * F.estride + off + (s2 & 15)];
+      x[f] = y;
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < DSS_MAX_FIELDS; ++f) {
+    if (f < a.nf) {
+      const DssField &F = a.f[f];
+      F.out[(size_t)ie * F.estride + (size_t)k * F.lstride + pt] = a.rsmp ? x[f] * r : x[f];
+    }
   }
 }
 
 void launch_dss(hipStream_t st, const DssArgs &a) {
-  const size_t nslab = (size_t)a.nelem * a.nlev;
-  const int blocks = (int)((nslab + 15) / 16);
-  hipLaunchKernelGGL(dss_kernel, dim3(blocks), dim3(256), 0, st, a);
+  if (a.nf <= 0) return;
+  hipLaunchKernelGGL(dss_kernel, dim3(group_blocks_x(a.nelem, a.nlev)), dim3(256), 0, st, a);
 }
 
 }  // namespace hx

@@ -1,27 +1,25 @@
 // Euler tracer step with quasi-monotone limiters (EulerStepFunctorImpl.hpp
 // and prim_advec_tracers_remap.cpp in the reference).
 //
-// Slab kernels: 16 lanes = one element at one level.  Kernels without
-// neighbour reads loop over all tracers of a slab (metric, dp and vstar are
-// loaded once per slab); kernels that gather neighbour copies (the DSS) loop
-// over a chunk of tracers so the neighbours' lines are still in L2 when they
-// are read.  One remap interval's tracer step is
-//   divdp_qmm   : divdp = div(vn0); stage-1 qmin/qmax
+// Slab kernels (16 lanes = one element at one level) loop over all tracers
+// of a slab, so metric, dp and vstar are loaded once per slab.  The exchange
+// kernels (DSS, biharmonic second half) are gathers and walk one field at a
+// time in level groups (group_slab) so neighbour copies come from L2.  One
+// remap interval's tracer step is
+//   divdp_qmm  : divdp = div(vn0); stage-1 qmin/qmax
 //   nbr_minmax
-//   advect(1)   ; dss(1)  (extra field divdp_proj)
-//   advect(2)   (stage-2 qmin/qmax update fused: no exchange at this stage)
-//   dss(2)      (extra field eta_dot_dpdn; fused: stage-3 qmin/qmax and the
-//                first laplacian of the tracer biharmonic)
-//   bih_post    (DSS of the first laplacian, second laplacian, scaling)
+//   advect(1)  ; dss           (extra field divdp_proj)
+//   advect(2)  (stage-2 qmin/qmax update fused: no exchange at this stage)
+//   dss        (extra field eta_dot_dpdn)
+//   qmm_lap    : stage-3 qmin/qmax and the first laplacian of the biharmonic
+//   bih_post   : DSS of the first laplacian, second laplacian, scaling
 //   nbr_minmax
-//   advect(3)   ; dss(3) (extra field omega_p; fused SSP average)
+//   advect(3)  ; dss + SSP average (extra field omega_p)
 // Limiter sums run over the 16 points in index order (slab_sum_serial) to
 // round like the reference's serial reductions.
 #include "kernels.hpp"
 
 namespace hx {
-
-constexpr int QCHUNK = 8;
 
 __device__ __forceinline__ bool slab_ie_k(size_t nslab_total, int nlev, size_t &slab, int &ie, int &k, int &pt) {
   const int b = xcd_block(blockIdx.x, gridDim.x);
@@ -30,22 +28,6 @@ __device__ __forceinline__ bool slab_ie_k(size_t nslab_total, int nlev, size_t &
   if (slab >= nslab_total) return false;
   ie = (int)(slab / nlev);
   k = (int)(slab % nlev);
-  return true;
-}
-
-// (tracer chunk, slab) decomposition with the chunk slowest
-__device__ __forceinline__ bool chunk_slab(const EulerArgs &a, int &q0, int &q1, size_t &slab, int &ie, int &k,
-                                           int &pt) {
-  const int nb = (int)(((size_t)a.nelem * a.nlev + 15) / 16);
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int chunk = lb / nb, sb = lb % nb;
-  q0 = chunk * QCHUNK;
-  q1 = q0 + QCHUNK < a.qsize ? q0 + QCHUNK : a.qsize;
-  slab = (size_t)sb * 16 + (threadIdx.x >> 4);
-  pt = threadIdx.x & 15;
-  if (slab >= (size_t)a.nelem * a.nlev) return false;
-  ie = (int)(slab / a.nlev);
-  k = (int)(slab % a.nlev);
   return true;
 }
 
@@ -98,14 +80,16 @@ __global__ __launch_bounds__(256) void euler_divdp_qmm_kernel(EulerArgs a) {
   }
 }
 
-// min/max of qlim over the element and its (up to 8) neighbours (:487-507)
+// min/max of qlim over the element and its (up to 8) neighbours (:487-507).
+// One thread per (tracer, element, level), tracer slowest, so a neighbour's
+// qlim row is still in L2 when the next element row reads it.
 __global__ __launch_bounds__(256) void euler_nbr_minmax_kernel(EulerArgs a) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int L = a.nlev, Q = a.qsize;
   if (i >= (size_t)a.nelem * Q * L) return;
   const int k = (int)(i % L);
-  const int q = (int)((i / L) % Q);
-  const int ie = (int)(i / ((size_t)L * Q));
+  const int ie = (int)((i / L) % a.nelem);
+  const int q = (int)(i / ((size_t)L * a.nelem));
   const size_t own = (((size_t)ie * Q + q) * 2) * L + k;
   double mn = a.qlim[own], mx = a.qlim[own + L];
   for (int c = 0; c < 8; ++c) {
@@ -242,86 +226,103 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
   }
 }
 
-// DSS of a chunk of tracers plus (chunk 0) the extra field.
-//   mode 0: plain
-//   mode 1: stage 2: also stage-3 compute_dp/qmin_qmax and the first
-//           laplacian of the tracer biharmonic (:196-267) -> qbih_pre
-//   mode 2: stage 3: SSP average (Qdp(n0) + 2 Qdp(np1)) / 3 (:379-403)
-__global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int mode, double dt3) {
-  int q0, q1, ie, k, pt;
-  size_t slab;
-  if (!chunk_slab(a, q0, q1, slab, ie, k, pt)) return;
-  const int L = a.nlev, Q = a.qsize;
+// DSS of tracers [QB*blockIdx.y, QB*blockIdx.y + QB) and, in the first
+// tracer batch, of the extra field.  avg: stage 3's SSP average
+// (Qdp(n0) + 2 Qdp(np1)) / 3 (:379-403) of the assembled tracer value.
+// All gathers of a lane are issued before its stores.
+constexpr int QB = 8;
+__global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int avg) {
+  int ie, k, pt;
+  if (!group_slab(a.nelem, a.nlev, ie, k, pt)) return;
+  const int L = a.nlev, Q = a.qsize, q0 = blockIdx.y * QB;
   const int *src = a.src + ((size_t)ie * 16 + pt) * 3;
   const int s0 = src[0], s1 = src[1], s2 = src[2];
   const double r = a.m.rspheremp[(size_t)ie * 16 + pt];
-  const size_t es = (size_t)Q * L * 16;
-  const size_t e3 = (size_t)L * 16, so = (size_t)k * 16;
-  ElemLane e;
-  DvvLane d;
-  double dp3 = 0, wdp = 0;
-  if (mode == 1) {
-    load_elem_light(a.m, ie, pt, e);
-    load_dvv(a.m.dvv, pt, d);
-    // stage 3: rhs_multiplier 2, divdp_proj final after stage 1
-    dp3 = a.derived_dp[(size_t)ie * e3 + so + pt] - (2.0 * dt3) * a.divdp_proj[(size_t)ie * e3 + so + pt];
-    wdp = a.dpdiss_ave[(size_t)ie * e3 + so + pt];
-  }
-  for (int q = q0; q < q1; ++q) {
-    const size_t off = ((size_t)q * L + k) * 16;
-    double x = gather(a.qdp_pre, es, off, ie, pt, s0, s1, s2);
-    x *= r;
-    if (mode == 2) x = (a.qdp_avg_n0[(size_t)ie * es + off + pt] + (3.0 - 1) * x) / 3.0;
-    a.qdp_np1[(size_t)ie * es + off + pt] = x;
-    if (mode == 1) {
-      const double qv = x / dp3;
-      const double mn = slab_min(qv), mx = slab_max(qv);
-      if (pt == 0) {
-        double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
-        ql[k] = mn;
-        ql[L + k] = mx;
-      }
-      double y = qv;
-      if (a.nu_p > 0) y = y * wdp / a.dp0[k];
-      a.qbih_pre[(size_t)ie * es + off + pt] = sph_laplace(e, d, y);
+  const size_t es = (size_t)Q * L * 16, e3 = (size_t)L * 16, so = (size_t)k * 16;
+  double x[QB], avg0[QB], xf = 0;
+#pragma unroll
+  for (int i = 0; i < QB; ++i) {
+    if (q0 + i < Q) {
+      const size_t off = ((size_t)(q0 + i) * L + k) * 16;
+      x[i] = gather(a.qdp_pre, es, off, ie, pt, s0, s1, s2);
+      avg0[i] = avg ? a.qdp_avg_n0[(size_t)ie * es + off + pt] : 0.0;
     }
   }
-  if (q0 == 0) {
-    double x = gather(a.fdss_pre, e3, so, ie, pt, s0, s1, s2);
-    x *= r;
-    a.fdss[(size_t)ie * e3 + so + pt] = x;
+  if (q0 == 0) xf = gather(a.fdss_pre, e3, so, ie, pt, s0, s1, s2);
+#pragma unroll
+  for (int i = 0; i < QB; ++i) {
+    if (q0 + i < Q) {
+      const size_t off = ((size_t)(q0 + i) * L + k) * 16;
+      double y = x[i] * r;
+      if (avg) y = (avg0[i] + (3.0 - 1) * y) / 3.0;
+      a.qdp_np1[(size_t)ie * es + off + pt] = y;
+    }
   }
+  if (q0 == 0) a.fdss[(size_t)ie * e3 + so + pt] = xf * r;
 }
 
-// biharmonic post (:216-310): DSS of the first laplacian (with rspheremp),
-// second laplacian, times -rhs_viss*dt*nu_q*dp0/spheremp
-__global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
-  int q0, q1, ie, k, pt;
+// Stage 3 compute_dp + compute_qmin_qmax (rhs_multiplier 2, :406-485) and the
+// first half of the tracer biharmonic (:196-267): laplace(Q * dpdiss_ave / dp0)
+// for every tracer of an (element, level) slab.
+__global__ __launch_bounds__(256) void euler_qmm_lap_kernel(EulerArgs a) {
   size_t slab;
-  if (!chunk_slab(a, q0, q1, slab, ie, k, pt)) return;
+  int ie, k, pt;
+  if (!slab_ie_k((size_t)a.nelem * a.nlev, a.nlev, slab, ie, k, pt)) return;
   const int L = a.nlev, Q = a.qsize;
   ElemLane e;
   load_elem_light(a.m, ie, pt, e);
   DvvLane d;
   load_dvv(a.m.dvv, pt, d);
+  const size_t s = slab * 16 + pt;
+  const double rhsmdt = a.rhs_multiplier * a.dt;
+  const double dp = a.derived_dp[s] - rhsmdt * a.divdp_proj[s];
+  const double wdp = a.dpdiss_ave[s];
+  const double dp0 = a.dp0[k];
+  for (int q = 0; q < Q; ++q) {
+    const size_t qi = (((size_t)ie * Q + q) * L + k) * 16 + pt;
+    const double x = a.qdp_n0[qi] / dp;
+    const double mn = slab_min(x), mx = slab_max(x);
+    if (pt == 0) {
+      double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
+      ql[k] = mn;
+      ql[L + k] = mx;
+    }
+    double y = x;
+    if (a.nu_p > 0) y = y * wdp / dp0;
+    a.qbih_pre[qi] = sph_laplace(e, d, y);
+  }
+}
+
+// biharmonic, second half (:216-310): DSS of the first laplacian (with
+// rspheremp), second laplacian, times -rhs_viss*dt*nu_q*dp0/spheremp, for
+// tracers [QB*blockIdx.y, QB*blockIdx.y + QB).
+__global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
+  int ie, k, pt;
+  if (!group_slab(a.nelem, a.nlev, ie, k, pt)) return;
+  const int L = a.nlev, Q = a.qsize, q0 = blockIdx.y * QB;
   const int *src = a.src + ((size_t)ie * 16 + pt) * 3;
   const int s0 = src[0], s1 = src[1], s2 = src[2];
   const size_t es = (size_t)Q * L * 16;
+  double x[QB];
+#pragma unroll
+  for (int i = 0; i < QB; ++i)
+    if (q0 + i < Q) x[i] = gather(a.qbih_pre, es, ((size_t)(q0 + i) * L + k) * 16, ie, pt, s0, s1, s2);
+  ElemLane e;
+  load_elem_light(a.m, ie, pt, e);
+  DvvLane d;
+  load_dvv(a.m.dvv, pt, d);
   const double f = -a.rhs_viss * a.dt * a.nu_q;
-  for (int q = q0; q < q1; ++q) {
-    const size_t off = ((size_t)q * L + k) * 16;
-    double x = gather(a.qbih_pre, es, off, ie, pt, s0, s1, s2);
-    x *= e.rsmp;
-    const double lap = sph_laplace(e, d, x);
-    a.qbih[(size_t)ie * es + off + pt] = (f * a.dp0[k] * lap / e.sph);
+  const double dp0 = a.dp0[k];
+#pragma unroll
+  for (int i = 0; i < QB; ++i) {
+    if (q0 + i < Q) {
+      const double lap = sph_laplace(e, d, x[i] * e.rsmp);
+      a.qbih[(size_t)ie * es + ((size_t)(q0 + i) * L + k) * 16 + pt] = (f * dp0 * lap / e.sph);
+    }
   }
 }
 
 static int slab_blocks(const EulerArgs &a) { return (int)(((size_t)a.nelem * a.nlev + 15) / 16); }
-static int chunk_blocks(const EulerArgs &a) {
-  const int nch = (a.qsize + QCHUNK - 1) / QCHUNK;
-  return slab_blocks(a) * (nch > 0 ? nch : 1);
-}
 
 void launch_euler_divdp_qmm(hipStream_t st, const EulerArgs &a) {
   hipLaunchKernelGGL(euler_divdp_qmm_kernel, dim3(slab_blocks(a)), dim3(256), 0, st, a);
@@ -336,12 +337,19 @@ void launch_euler_advect(hipStream_t st, const EulerArgs &a, int stage) {
   hipLaunchKernelGGL(euler_advect_kernel, dim3(slab_blocks(a)), dim3(256), 0, st, a, stage);
 }
 
-void launch_euler_dss(hipStream_t st, const EulerArgs &a, int mode, double dt3) {
-  hipLaunchKernelGGL(euler_dss_kernel, dim3(chunk_blocks(a)), dim3(256), 0, st, a, mode, dt3);
+void launch_euler_dss(hipStream_t st, const EulerArgs &a, int avg) {
+  const int nq = a.qsize > 0 ? (a.qsize + QB - 1) / QB : 1;
+  hipLaunchKernelGGL(euler_dss_kernel, dim3(group_blocks_x(a.nelem, a.nlev), nq), dim3(256), 0, st, a, avg);
+}
+
+void launch_euler_qmm_lap(hipStream_t st, const EulerArgs &a) {
+  hipLaunchKernelGGL(euler_qmm_lap_kernel, dim3(slab_blocks(a)), dim3(256), 0, st, a);
 }
 
 void launch_euler_bih_post(hipStream_t st, const EulerArgs &a) {
-  hipLaunchKernelGGL(euler_bih_post_kernel, dim3(chunk_blocks(a)), dim3(256), 0, st, a);
+  if (a.qsize <= 0) return;
+  hipLaunchKernelGGL(euler_bih_post_kernel, dim3(group_blocks_x(a.nelem, a.nlev), (a.qsize + QB - 1) / QB), dim3(256),
+                     0, st, a);
 }
 
 }  // namespace hx

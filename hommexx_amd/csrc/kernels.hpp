@@ -89,7 +89,8 @@ void launch_hv_cycle(hipStream_t st, const HvArgs &a);
 void launch_euler_divdp_qmm(hipStream_t st, const EulerArgs &a);
 void launch_euler_nbr_minmax(hipStream_t st, const EulerArgs &a);
 void launch_euler_advect(hipStream_t st, const EulerArgs &a, int stage);
-void launch_euler_dss(hipStream_t st, const EulerArgs &a, int mode, double dt3);
+void launch_euler_dss(hipStream_t st, const EulerArgs &a, int avg);
+void launch_euler_qmm_lap(hipStream_t st, const EulerArgs &a);
 void launch_euler_bih_post(hipStream_t st, const EulerArgs &a);
 void launch_state_forcing(hipStream_t st, int nelem, int nlev, double dt, const double *ft, const double *fm,
                           double *t, double *v);
@@ -102,6 +103,29 @@ void launch_zero_int(hipStream_t st, int *p);
 __device__ __forceinline__ int xcd_block(int b, int n) {
   const int q = n >> 3, r = n & 7, x = b & 7;
   return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+// Exchange (gather) kernels walk one field at a time (blockIdx.y) through
+// groups of GL = 8 levels, element by element: a 256-thread block is two
+// elements x 8 levels, 16 lanes per (element, level).  The neighbour copies a
+// block reads were loaded by blocks a few hundred positions earlier in the
+// same level group (a few hundred KB), so they are still in the XCD's L2.
+constexpr int GL = 8;
+inline int group_blocks_x(int nelem, int nlev) {
+  const long long ns = (long long)nelem * ((nlev + GL - 1) / GL) * GL;
+  const int nb = (int)((ns + 15) / 16);
+  return (nb + 7) & ~7;  // multiple of 8: blockIdx.x % 8 is then the XCD
+}
+__device__ __forceinline__ bool group_slab(int nelem, int nlev, int &ie, int &k, int &pt) {
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const long long s = (long long)lb * 16 + (threadIdx.x >> 4);
+  pt = threadIdx.x & 15;
+  const int kk = (int)(s & (GL - 1));
+  const long long r = s >> 3;
+  ie = (int)(r % nelem);
+  const int g = (int)(r / nelem);
+  k = g * GL + kk;
+  return g < (nlev + GL - 1) / GL && k < nlev;
 }
 
 }  // namespace hx

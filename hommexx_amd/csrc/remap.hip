@@ -2,14 +2,15 @@
 // reference hybrid levels with monotone PPM (RemapFunctor.hpp:226-331,
 // PpmRemap.hpp:78-673 in the reference), plus update_q (prim_driver.cpp:171-206).
 //
-// One lane = one (GLL column, field).  The reference builds per-column arrays
-// (pio, pin, ppmdx[10], kid, z2, ao, dma, ai, parabola coefficients) and then
-// remaps; here one top-to-bottom sweep produces the same numbers with O(1)
-// state: source cell j's parabola needs ai(j-1), ai(j), which need
-// dma(j-1..j+1) and a, dx up to j+2, so a four-cell window suffices, and the
-// target interface k+1 lies in source cell kid[k] >= k-1 (kid is found by the
-// same forward search).  Every value is computed with the reference's
-// expression and operand order.
+// The reference builds per-column arrays (pio, pin, ppmdx[10], kid, z2, ao,
+// dma, ai, parabola coefficients) and then remaps; here one top-to-bottom
+// sweep produces the same numbers with O(1) state: source cell j's parabola
+// needs ai(j-1), ai(j), which need dma(j-1..j+1) and a, dx up to j+2, so a
+// four-cell window suffices, and the target interface k+1 lies in source cell
+// kid[k] >= k-1 (kid is found by the same forward search).  Every value is
+// computed with the reference's expression and operand order; grid factors
+// that depend on the thicknesses only (the reference's ppmdx rows) are
+// evaluated once per column and shared by a batch of fields.
 #include "kernels.hpp"
 
 namespace hx {
@@ -32,12 +33,6 @@ __global__ __launch_bounds__(256) void remap_thickness_kernel(RemapArgs a) {
   if (bad) atomicOr(a.error, 1);
 }
 
-struct Cell {
-  double c0, c1, c2;  // parabola: const, linear, square
-  double mass;        // exclusive prefix of the remapped quantity at this cell
-  double dx;          // source thickness
-};
-
 __device__ __forceinline__ double dmin3(double a, double b, double c) {
   const double bc = b < c ? b : c;
   return a < bc ? a : bc;
@@ -59,17 +54,20 @@ __device__ __forceinline__ double ppm_dma(double am1, double a0, double a1, doub
   return 0.0;
 }
 
-// ai(j) from a(j), a(j+1), dma(j), dma(j+1), dx(j-1..j+2)
-__device__ __forceinline__ double ppm_ai(double a0, double a1, double dma0, double dma1, double xm1, double x0,
-                                         double x1, double x2) {
-  const double g3 = x0 / (x0 + x1);
-  const double g4 = 1.0 / (xm1 + x0 + x1 + x2);
-  const double g5 = (2.0 * x0 * x1) / (x0 + x1);
-  const double g6 = (xm1 + x0) / (2.0 * x0 + x1);
-  const double g7 = (x2 + x1) / (2.0 * x1 + x0);
-  const double g8 = x0 * (xm1 + x0) / (2.0 * x0 + x1);
-  const double g9 = x1 * (x1 + x2) / (x0 + 2.0 * x1);
-  return a0 + g3 * (a1 - a0) + g4 * (g5 * (g6 - g7) * (a1 - a0) - g8 * dma1 + g9 * dma0);
+// the same with the grid factors g0, g1, g2 precomputed
+__device__ __forceinline__ double ppm_dma_g(double am1, double a0, double a1, double g0, double g1, double g2) {
+  if ((a1 - a0) * (a0 - am1) > 0.0) {
+    const double da = g0 * (g1 * (a1 - a0) + g2 * (a0 - am1));
+    return dmin3(fabs(da), 2.0 * fabs(a0 - am1), 2.0 * fabs(a1 - a0)) * copysign(1.0, da);
+  }
+  return 0.0;
+}
+
+// ai(j) from a(j), a(j+1), dma(j), dma(j+1) and the grid factors of
+// dx(j-1..j+2): g3, g4, g56 = g5*(g6-g7), g8, g9 (the product associates
+// left to right, so g5*(g6-g7)*(a1-a0) == g56*(a1-a0) exactly)
+__device__ __forceinline__ double ppm_ai_g(double a0, double a1, double dma0, double dma1, const double *g) {
+  return a0 + g[0] * (a1 - a0) + g[1] * (g[2] * (a1 - a0) - g[3] * dma1 + g[4] * dma0);
 }
 
 __device__ __forceinline__ void ppm_parabola(double al, double ar, double a, double &c0, double &c1, double &c2) {
@@ -84,43 +82,45 @@ __device__ __forceinline__ void ppm_parabola(double al, double ar, double a, dou
   c2 = 3.0 * (-2.0 * a + (al + ar));
 }
 
-// One (column, field).  field 0,1: v components, 2: T, >= 3: tracer field-3.
-__global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
+// One lane = one GLL column and a batch of RFB fields (field 0,1: v
+// components, 2: T, >= 3: tracer field-3).  A wave is 64 columns (four
+// elements) of one field batch, so the field kinds are wave-uniform.  The
+// grid factors, the target search (kid, z2) and the control flow of the sweep
+// depend on the column only and are shared by the batch.
+template <int RFB>  // fields per lane
+__global__ __launch_bounds__(256) void remap_kernel(RemapArgs a, int nfb) {
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int w = lb * 4 + (threadIdx.x >> 6);  // global wave
+  const int eg = w / nfb, fb = w % nfb;
+  const int ie = eg * 4 + ((threadIdx.x >> 4) & 3);
+  if (ie >= a.nelem) return;
+  const int pt = threadIdx.x & 15;
   const int nf = 3 + a.qsize;
-  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  // 16 lanes = the 16 columns of one element for one field, so each level is
-  // one coalesced 128-byte access; the fields of an element follow each other
-  // (the element's dp3d columns stay in cache across its fields)
-  const int pt = (int)(tid & 15);
-  const size_t rest = tid >> 4;
-  const int f = (int)(rest % nf);
-  const size_t ie_s = rest / nf;
-  if (ie_s >= (size_t)a.nelem) return;
-  const int ie = (int)ie_s;
   const size_t col = (size_t)ie * 16 + pt;
   const int L = a.nlev;
   const double *src = a.dp + (size_t)ie * L * 16 + pt;
-  double *var;
-  size_t vs = 16;
-  if (f < 2) {
-    var = a.v + (size_t)ie * L * 32 + f * 16 + pt;
-    vs = 32;
-  } else if (f == 2) {
-    var = a.t + (size_t)ie * L * 16 + pt;
-  } else {
-    var = a.qdp + (((size_t)ie * a.qsize + (f - 3)) * L) * 16 + pt;
+  double *var[RFB];
+  double *Qout[RFB];
+  size_t vs[RFB];
+  bool on[RFB], state[RFB];
+#pragma unroll
+  for (int i = 0; i < RFB; ++i) {
+    const int f = fb * RFB + i;
+    on[i] = f < nf;
+    state[i] = f < 3;
+    vs[i] = f < 2 ? 32 : 16;
+    Qout[i] = nullptr;
+    if (f < 2) var[i] = a.v + (size_t)ie * L * 32 + f * 16 + pt;
+    else if (f == 2) var[i] = a.t + (size_t)ie * L * 16 + pt;
+    else if (f < nf) {
+      var[i] = a.qdp + (((size_t)ie * a.qsize + (f - 3)) * L) * 16 + pt;
+      if (a.Q) Qout[i] = a.Q + (((size_t)ie * a.qsize + (f - 3)) * L) * 16 + pt;
+    } else var[i] = nullptr;
   }
-  double *Qout = (f >= 3 && a.Q) ? a.Q + (((size_t)ie * a.qsize + (f - 3)) * L) * 16 + pt : nullptr;
-  const bool state = f < 3;
   const double ps = a.ps[col];
   const int alg = a.remap_alg;
   auto tgt_of = [&](int k) {
     return (a.hyai[k + 1] - a.hyai[k]) * a.ps0 + (a.hybi[k + 1] - a.hybi[k]) * ps;
-  };
-  // extensive value of physical cell c (states are multiplied by dp3d first)
-  auto load_var = [&](int c, double dxc) {
-    const double x = var[(size_t)c * vs];
-    return state ? x * dxc : x;
   };
   // pio(L): serial sum of the source thicknesses
   double pio_L = 0.0;
@@ -128,27 +128,44 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
 
   // ------------------------------------------------------------ window
   // Fortran-style cell indices j = 1..L, mirrored (or constant) ghosts.
-  // (Xm1, X0, X1, X2) = dx(j-1..j+2), (Am1..A2) = a(j-1..j+2), V0..V2 = var(j..j+2)
+  // (Xm1, X0, X1, X2) = dx(j-1..j+2); per field (Am1..A2) = a(j-1..j+2),
+  // V0..V2 = var(j..j+2)
   const double s0 = src[0], s1 = src[16];
-  const double v0 = load_var(0, s0), v1 = load_var(1, s1);
-  const double a1 = v0 / s0, a2 = v1 / s1;
   double Xm1 = s1, X0 = s0, X1 = s0, X2 = s1;
-  double Am1 = alg == 3 ? a1 : a2, A0 = a1, A1 = a1, A2 = a2;
-  double V0 = 0.0, V1 = v0, V2 = v1;  // var(0) unused
-  double dma_j = ppm_dma(Am1, A0, A1, Xm1, X0, X1);  // dma(0)
-  double ai_prev = 0.0;
-  double var_prefix = 0.0;  // sum of var over physical cells < c
+  double Am1[RFB], A0[RFB], A1[RFB], A2[RFB], V0[RFB], V1[RFB], V2[RFB];
+  double dma_j[RFB], ai_prev[RFB], var_prefix[RFB], mass_prev[RFB];
+  double cc0[RFB], cc1[RFB], cc2[RFB];
+  double g0, g1, g2;  // dma factors of the current window
+  {
+    g0 = X0 / (Xm1 + X0 + X1);
+    g1 = (2.0 * Xm1 + X0) / (X0 + X1);
+    g2 = (X0 + 2.0 * X1) / (Xm1 + X0);
+  }
+#pragma unroll
+  for (int i = 0; i < RFB; ++i) {
+    if (!on[i]) continue;
+    const double x0 = var[i][0], x1 = var[i][vs[i]];
+    const double v0 = state[i] ? x0 * s0 : x0, v1 = state[i] ? x1 * s1 : x1;
+    const double a1 = v0 / s0, a2 = v1 / s1;
+    Am1[i] = alg == 3 ? a1 : a2;
+    A0[i] = a1;
+    A1[i] = a1;
+    A2[i] = a2;
+    V0[i] = 0.0;
+    V1[i] = v0;
+    V2[i] = v1;
+    dma_j[i] = ppm_dma_g(Am1[i], A0[i], A1[i], g0, g1, g2);  // dma(0)
+    ai_prev[i] = 0.0;
+    var_prefix[i] = 0.0;
+    mass_prev[i] = 0.0;
+  }
 
   // ------------------------------------------------------------ targets
-  int k = 0;                 // next target level
-  int kid = -2;              // kid of target k (-2: not yet computed)
-  double z2 = 0.0;
-  double pin_acc = 0.0;      // serial prefix of the target thicknesses
-  int si = 0;                // next source interface to test
+  int k = 0;     // next target level
+  int kid = -2;  // kid of target k (-2: not yet computed)
+  double z2 = 0.0, pin_acc = 0.0;
+  int si = 0;
   double pio_si = 0.0, pio_m1 = 0.0, pio_m2 = 0.0;  // pio(si), pio(si-1), pio(si-2)
-  double mass_prev = 0.0;
-  Cell prev{0, 0, 0, 0, 0};
-  bool have_prev = false;
 
   auto advance_si = [&]() {
     pio_m2 = pio_m1;
@@ -176,43 +193,79 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
     kid = kk - 1;
     z2 = kid >= 0 ? (pin_k1 - (plo + phi) * 0.5) / src[(size_t)kid * 16] : 0.0;
   };
-  auto finish = [&](const Cell &cc) {
-    const double integral = integrate_parabola(cc.c2, cc.c1, cc.c0, -0.5, z2);
-    const double mass = cc.mass + integral * cc.dx;
-    const double out = k > 0 ? mass - mass_prev : mass;
-    mass_prev = mass;
+  // finish target k, whose upper interface lies in the current cell
+  auto finish = [&](double dx) {
     const double tk = tgt_of(k);
-    if (state) {
-      var[(size_t)k * vs] = out / tk;
-    } else {
-      var[(size_t)k * vs] = out;
-      if (Qout) Qout[(size_t)k * 16] = out / tk;
+#pragma unroll
+    for (int i = 0; i < RFB; ++i) {
+      if (!on[i]) continue;
+      const double integral = integrate_parabola(cc2[i], cc1[i], cc0[i], -0.5, z2);
+      const double mass = var_prefix[i] + integral * dx;
+      const double out = k > 0 ? mass - mass_prev[i] : mass;
+      mass_prev[i] = mass;
+      if (state[i]) {
+        var[i][(size_t)k * vs[i]] = out / tk;
+      } else {
+        var[i][(size_t)k * vs[i]] = out;
+        if (Qout[i]) Qout[i][(size_t)k * 16] = out / tk;
+      }
     }
     ++k;
     kid = -2;
   };
 
   // ------------------------------------------------------------ sweep
+  // cell j+3 enters the window at iteration j; its dx and field values are
+  // loaded one iteration ahead so the load latency hides behind a sweep step
+  double px = 0.0, pv[RFB];
+  auto fetch = [&](int m, double &x, double *v) {
+    if (m <= L) {
+      x = src[(size_t)(m - 1) * 16];
+#pragma unroll
+      for (int i = 0; i < RFB; ++i)
+        if (on[i]) v[i] = var[i][(size_t)(m - 1) * vs[i]];
+    }
+  };
+  fetch(3, px, pv);
   for (int j = 0; j <= L; ++j) {
-    const double dma_j1 = ppm_dma(A0, A1, A2, X0, X1, X2);               // dma(j+1)
-    const double ai_j = ppm_ai(A0, A1, dma_j, dma_j1, Xm1, X0, X1, X2);  // ai(j)
+    double nx = 0.0, nv[RFB];
+    fetch(j + 4, nx, nv);
+    // dma(j+1) factors: cells j..j+2; ai(j) factors: cells j-1..j+2
+    const double h0 = X1 / (X0 + X1 + X2);
+    const double h1 = (2.0 * X0 + X1) / (X1 + X2);
+    const double h2 = (X1 + 2.0 * X2) / (X0 + X1);
+    double gai[5];
+    gai[0] = X0 / (X0 + X1);
+    gai[1] = 1.0 / (Xm1 + X0 + X1 + X2);
+    gai[2] = (2.0 * X0 * X1) / (X0 + X1) * ((Xm1 + X0) / (2.0 * X0 + X1) - (X2 + X1) / (2.0 * X1 + X0));
+    gai[3] = X0 * (Xm1 + X0) / (2.0 * X0 + X1);
+    gai[4] = X1 * (X1 + X2) / (X0 + 2.0 * X1);
+    double dma_j1[RFB], ai_j[RFB];
+#pragma unroll
+    for (int i = 0; i < RFB; ++i) {
+      if (!on[i]) continue;
+      dma_j1[i] = ppm_dma_g(A0[i], A1[i], A2[i], h0, h1, h2);            // dma(j+1)
+      ai_j[i] = ppm_ai_g(A0[i], A1[i], dma_j[i], dma_j1[i], gai);       // ai(j)
+    }
     if (j >= 1) {
       const int c = j - 1;  // physical source cell, Fortran cell j
-      Cell cc;
-      ppm_parabola(ai_prev, ai_j, A0, cc.c0, cc.c1, cc.c2);
-      if (alg == 2 && (c <= 1 || c >= L - 2)) {
-        cc.c0 = A0;
-        cc.c1 = 0.0;
-        cc.c2 = 0.0;
+#pragma unroll
+      for (int i = 0; i < RFB; ++i) {
+        if (!on[i]) continue;
+        ppm_parabola(ai_prev[i], ai_j[i], A0[i], cc0[i], cc1[i], cc2[i]);
+        if (alg == 2 && (c <= 1 || c >= L - 2)) {
+          cc0[i] = A0[i];
+          cc1[i] = 0.0;
+          cc2[i] = 0.0;
+        }
       }
-      cc.dx = X0;
-      cc.mass = var_prefix;
+      // kid is non-decreasing in k (each search continues from the last
+      // interface reached, starting no lower than k), so every target is
+      // finished while the sweep is at its own cell kid(k)
       while (k < L) {
         if (kid == -2) compute_kid();
         if (kid == c) {
-          finish(cc);
-        } else if (kid == c - 1 && have_prev) {
-          finish(prev);
+          finish(X0);
         } else if (kid < c) {
           atomicOr(a.error, 2);  // deformation beyond one layer: not supported
           return;
@@ -220,29 +273,37 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
           break;
         }
       }
-      prev = cc;
-      have_prev = true;
-      var_prefix += V0;
+#pragma unroll
+      for (int i = 0; i < RFB; ++i)
+        if (on[i]) var_prefix[i] += V0[i];
     }
     // shift: bring in cell j+3
     const int m = j + 3;
-    double xn, an, vn = 0.0;
-    if (m <= L) {
-      xn = src[(size_t)(m - 1) * 16];
-      vn = load_var(m - 1, xn);
-      an = vn / xn;
-    } else if (m == L + 1) {
-      xn = X2;  // dx(L+1) = dx(L)
-      an = A2;  // a(L+1) = a(L)
-    } else {
-      xn = X0;                 // dx(L+2) = dx(L-1)
-      an = alg == 3 ? A1 : A0;  // a(L+2) = a(L) or a(L-1)
+    double xn;
+    if (m <= L) xn = px;
+    else if (m == L + 1) xn = X2;  // dx(L+1) = dx(L)
+    else xn = X0;                  // dx(L+2) = dx(L-1)
+#pragma unroll
+    for (int i = 0; i < RFB; ++i) {
+      if (!on[i]) continue;
+      double an, vn = 0.0;
+      if (m <= L) {
+        const double x = pv[i];
+        vn = state[i] ? x * xn : x;
+        an = vn / xn;
+      } else if (m == L + 1) {
+        an = A2[i];  // a(L+1) = a(L)
+      } else {
+        an = alg == 3 ? A1[i] : A0[i];  // a(L+2) = a(L) or a(L-1)
+      }
+      Am1[i] = A0[i]; A0[i] = A1[i]; A1[i] = A2[i]; A2[i] = an;
+      V0[i] = V1[i]; V1[i] = V2[i]; V2[i] = vn;
+      ai_prev[i] = ai_j[i];
+      dma_j[i] = dma_j1[i];
+      pv[i] = nv[i];
     }
     Xm1 = X0; X0 = X1; X1 = X2; X2 = xn;
-    Am1 = A0; A0 = A1; A1 = A2; A2 = an;
-    V0 = V1; V1 = V2; V2 = vn;
-    ai_prev = ai_j;
-    dma_j = dma_j1;
+    px = nx;
   }
   if (k < L) atomicOr(a.error, 2);
 }
@@ -255,8 +316,13 @@ void launch_zero_int(hipStream_t st, int *p) { hipLaunchKernelGGL(zero_int_kerne
 void launch_remap(hipStream_t st, const RemapArgs &a) {
   const size_t ncol = (size_t)a.nelem * 16;
   hipLaunchKernelGGL(remap_thickness_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, a);
-  const size_t n = ncol * (3 + a.qsize);
-  hipLaunchKernelGGL(remap_kernel, dim3((n + 255) / 256), dim3(256), 0, st, a);
+  // four fields per lane: the column work (grid factors, target search,
+  // control flow) is shared four ways at two waves per SIMD
+  constexpr int fb = 4;
+  const int nfb = (3 + a.qsize + fb - 1) / fb;
+  const int nwaves = ((a.nelem + 3) / 4) * nfb;
+  const int nb = (((nwaves + 3) / 4) + 7) & ~7;
+  hipLaunchKernelGGL(remap_kernel<fb>, dim3(nb), dim3(256), 0, st, a, nfb);
 }
 
 }  // namespace hx
