@@ -25,40 +25,59 @@ def build(force=False):
         subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
 
 
+def _bind(path):
+    """Load one build of the oracle and declare its entry points."""
+    L = ctypes.CDLL(path)
+    L.oracle_create.restype = _vp
+    L.oracle_create.argtypes = [ctypes.c_int] * 4
+    L.oracle_destroy.argtypes = [_vp]
+    L.oracle_set_params.restype = ctypes.c_int
+    L.oracle_set_params.argtypes = [_vp] + [ctypes.c_int] * 7 + [ctypes.c_double] * 6 + \
+        [ctypes.c_int] * 2 + [ctypes.c_double] + [ctypes.c_int] * 2
+    L.oracle_set_dvv.argtypes = [_vp, _dp]
+    L.oracle_set_2d.argtypes = [_vp] + [_dp] * 11
+    L.oracle_set_hvcoord.argtypes = [_vp, ctypes.c_double, _dp, _dp]
+    L.oracle_add_connection.restype = ctypes.c_int
+    L.oracle_add_connection.argtypes = [_vp] + [ctypes.c_int] * 8
+    L.oracle_set_states.argtypes = [_vp] + [_dp] * 5
+    L.oracle_get_states.argtypes = [_vp] + [_dp] * 7
+    L.oracle_set_time_level.argtypes = [_vp] + [ctypes.c_int] * 5
+    L.oracle_run_subcycle.restype = ctypes.c_int
+    L.oracle_run_subcycle.argtypes = [_vp, ctypes.c_double, _ip, _ip, _ip, _ip]
+    L.oracle_unit_gradient.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
+    L.oracle_unit_divergence.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.oracle_unit_vorticity.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.oracle_unit_laplace.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.oracle_unit_vlaplace.argtypes = [ctypes.c_int, ctypes.c_double] + [_dp] * 9
+    L.oracle_unit_limiter.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
+    L.oracle_unit_ppm.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]
+    L.oracle_unit_dss.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int]
+    L.oracle_num_threads.restype = ctypes.c_int
+    L.oracle_debug_stop.argtypes = [_vp, ctypes.c_int]
+    return L
+
+
 def lib():
+    """The oracle (default build)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
-        L = ctypes.CDLL(LIB_PATH)
-        L.oracle_create.restype = _vp
-        L.oracle_create.argtypes = [ctypes.c_int] * 4
-        L.oracle_destroy.argtypes = [_vp]
-        L.oracle_set_params.restype = ctypes.c_int
-        L.oracle_set_params.argtypes = [_vp] + [ctypes.c_int] * 7 + [ctypes.c_double] * 6 + \
-            [ctypes.c_int] * 2 + [ctypes.c_double] + [ctypes.c_int] * 2
-        L.oracle_set_dvv.argtypes = [_vp, _dp]
-        L.oracle_set_2d.argtypes = [_vp] + [_dp] * 11
-        L.oracle_set_hvcoord.argtypes = [_vp, ctypes.c_double, _dp, _dp]
-        L.oracle_add_connection.restype = ctypes.c_int
-        L.oracle_add_connection.argtypes = [_vp] + [ctypes.c_int] * 8
-        L.oracle_set_states.argtypes = [_vp] + [_dp] * 5
-        L.oracle_get_states.argtypes = [_vp] + [_dp] * 7
-        L.oracle_set_time_level.argtypes = [_vp] + [ctypes.c_int] * 5
-        L.oracle_run_subcycle.restype = ctypes.c_int
-        L.oracle_run_subcycle.argtypes = [_vp, ctypes.c_double, _ip, _ip, _ip, _ip]
-        L.oracle_unit_gradient.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
-        L.oracle_unit_divergence.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
-        L.oracle_unit_vorticity.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
-        L.oracle_unit_laplace.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
-        L.oracle_unit_vlaplace.argtypes = [ctypes.c_int, ctypes.c_double] + [_dp] * 9
-        L.oracle_unit_limiter.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
-        L.oracle_unit_ppm.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]
-        L.oracle_unit_dss.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int]
-        L.oracle_num_threads.restype = ctypes.c_int
-        L.oracle_debug_stop.argtypes = [_vp, ctypes.c_int]
-        _lib = L
+        _lib = _bind(LIB_PATH)
     return _lib
+
+
+_plants = {}
+
+
+def plant(n):
+    """The oracle built with planted mistake ``n`` (see ORACLE_PLANT in
+    preqx_oracle.c); used only to show that the pinning tests catch it."""
+    if n not in _plants:
+        path = os.path.join(_HERE, "plants", "liboracle_plant%d.so" % n)
+        subprocess.check_call(["make", "-s", "-C", _HERE, os.path.join("plants", os.path.basename(path))])
+        _plants[n] = _bind(path)
+    return _plants[n]
 
 
 def ptr(a):
@@ -73,8 +92,9 @@ def f64(a):
 class OracleSession:
     """Runs the reference step on the CPU for a :class:`hommexx_amd.problem.Problem`."""
 
-    def __init__(self, prob):
-        L = lib()
+    def __init__(self, prob, L=None):
+        L = lib() if L is None else L
+        self.L = L
         self.prob = prob
         p = prob.params
         self.h = L.oracle_create(prob.nelem, prob.nlev, prob.qsize, prob.qsize_d)
@@ -102,7 +122,7 @@ class OracleSession:
         self.tl = None
 
     def run(self, nsteps=1, dt=None):
-        L = lib()
+        L = self.L
         dt = self.prob.dt if dt is None else dt
         a = [ctypes.c_int() for _ in range(4)]
         for _ in range(nsteps):
@@ -117,13 +137,13 @@ class OracleSession:
         s = pr.state
         out = {k: np.zeros_like(s[k]) for k in ("v", "T", "dp3d", "Qdp", "Q", "ps_v")}
         out["omega_p"] = np.zeros((pr.nelem, pr.nlev, 4, 4))
-        lib().oracle_get_states(self.h, ptr(out["v"]), ptr(out["T"]), ptr(out["dp3d"]), ptr(out["Qdp"]),
+        self.L.oracle_get_states(self.h, ptr(out["v"]), ptr(out["T"]), ptr(out["dp3d"]), ptr(out["Qdp"]),
                                 ptr(out["Q"]), ptr(out["ps_v"]), ptr(out["omega_p"]))
         return out
 
     def close(self):
         if self.h:
-            lib().oracle_destroy(self.h)
+            self.L.oracle_destroy(self.h)
             self.h = None
 
     def __del__(self):

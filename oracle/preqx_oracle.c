@@ -23,6 +23,24 @@
  * CPU and with HOMMEXX_GPU_BFB_WITH_CPU on GPU).  Build with
  * -ffp-contract=off so no multiply-add is fused.
  *
+ * How each part is pinned (tests/test_oracle.py, tests/test_oracle_pinning.py):
+ *   sphere operators  the reference's golden vectors (bit-exact), closed forms
+ *                     on the sphere, sum(weak laplacian) == 0
+ *   DSS               continuity of every node, DSS(spheremp c) rspheremp == c,
+ *                     54 ne^2 + 2 unique nodes
+ *   CAAR              the steady JW jet stays balanced, converging with ne
+ *   hyperviscosity    damping of a Y2 harmonic at dt nu (6/a^2)^2 per subcycle
+ *   Euler step        constant mixing ratio kept; zonal tracer kept by the
+ *                     zonal jet (third-order convergence); limiter bounds,
+ *                     mass and 1-norm properties (preqx_ut.cpp:1335-1532)
+ *   PPM remap         identity, constants, column mass, monotone range,
+ *                     linear profiles exact in the interior
+ *   full step         dry and tracer mass conserved to 1e-14
+ * Each check is shown to fail on a build with one planted mistake
+ * (ORACLE_PLANT below).  The exact accumulation order of the DSS follows
+ * ConnectivityHelpers.hpp; property checks cannot see summation order, so
+ * that order is pinned by reading only.
+ *
  * Internal layout mirrors the reference C++ views: levels fastest,
  *   3-D scalar  [ie][igp][jgp][lev]
  *   2-D tensor  [ie][idim][jdim][igp][jgp] (flat copy of the Fortran arrays).
@@ -35,6 +53,14 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+
+/* Planted mistakes for the pinning tests (tests/test_oracle_pinning.py):
+ * built with -DORACLE_PLANT=n the oracle carries one deliberate error, which
+ * the invariant tests must detect.  The default build (0) has none. */
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+#define PLANT(n) (ORACLE_PLANT == (n))
 
 #define NP 4
 #define NP2 16
@@ -153,7 +179,7 @@ static void divergence_sphere(const Oracle *o, int ie, int nl, const double *v0,
       for (int k = 0; k < nl; ++k) {
         double dudx = 0, dvdy = 0;
         for (int kk = 0; kk < NP; ++kk) {
-          dudx += o->dvv[jg][kk] * AT(gv0, ig, kk, k);
+          dudx += (PLANT(8) ? o->dvv[kk][jg] : o->dvv[jg][kk]) * AT(gv0, ig, kk, k); /* plant 8: transposed dvv */
           dvdy += o->dvv[ig][kk] * AT(gv1, kk, jg, k);
         }
         AT(div, ig, jg, k) = (dudx + dvdy) * (1.0 / M2(metdet, ig, jg) * RREARTH);
@@ -178,12 +204,13 @@ static void divergence_sphere_update(const Oracle *o, int ie, double alpha, int 
       for (int k = 0; k < L; ++k) {
         double dudx = 0, dvdy = 0;
         for (int kk = 0; kk < NP; ++kk) {
-          dudx += o->dvv[jg][kk] * AT(gv0, ig, kk, k);
+          dudx += (PLANT(8) ? o->dvv[kk][jg] : o->dvv[jg][kk]) * AT(gv0, ig, kk, k); /* plant 8: transposed dvv */
           dvdy += o->dvv[ig][kk] * AT(gv1, kk, jg, k);
         }
         const double q0 = add_hv ? AT(qtens, ig, jg, k) : 0;
         AT(qtens, ig, jg, k) = (AT(qdp, ig, jg, k) +
-                                alpha * ((dudx + dvdy) * (1.0 / M2(metdet, ig, jg) * RREARTH)) + q0);
+                                (PLANT(5) ? -alpha : alpha) * ((dudx + dvdy) * (1.0 / M2(metdet, ig, jg) * RREARTH)) +
+                                q0); /* plant 5: flux-divergence sign */
       }
 }
 
@@ -373,6 +400,7 @@ static void dss(Oracle *o, int nf, double **fields, const size_t *estride, int n
         }
       for (int c = 4; c < NCONN; ++c) {
         if (o->conn_lid[ie * NCONN + c] < 0) continue; /* missing corner: nothing to add */
+        if (PLANT(6)) continue;                        /* planted: corner copies dropped */
         const double *buf = o->dss_buf + ((size_t)ie * NCONN + c) * slot;
         const int ig = CORNER_PT[c - 4][0], jg = CORNER_PT[c - 4][1];
         for (int lv = 0; lv < nl; ++lv) AT(dst, ig, jg, lv) += buf[(size_t)f * L + lv];
@@ -488,7 +516,9 @@ static void caar_element(Oracle *o, int ie, int nm1, int n0, int np1, int n0_qdp
   for (size_t i = 0; i < E3; ++i) {
     eg0[i] = RGAS * (tv[i] / p[i]) * g0[i];
     eg1[i] = RGAS * (tv[i] / p[i]) * g1[i];
+    if (PLANT(3)) { eg0[i] = -eg0[i]; eg1[i] = -eg1[i]; } /* planted: pressure-gradient sign */
     ephi[i] = 0.5 * (v0n[i] * v0n[i] + v1n[i] * v1n[i]) + phi[i];
+    if (PLANT(2)) ephi[i] = phi[i]; /* planted: kinetic-energy gradient dropped */
   }
   gradient_sphere_update(o, ie, ephi, eg0, eg1);
   vorticity_sphere(o, ie, L, v0n, v1n, vort, b0, b1);
@@ -497,7 +527,8 @@ static void caar_element(Oracle *o, int ie, int nm1, int n0, int np1, int n0_qdp
   for (int pt = 0; pt < NP2; ++pt)
     for (int k = 0; k < L; ++k) {
       const size_t i = (size_t)pt * L + k;
-      vort[i] += fcor[pt];
+      if (PLANT(1)) vort[i] -= fcor[pt]; /* planted: Coriolis sign */
+      else vort[i] += fcor[pt];
       eg0[i] *= -1;
       eg0[i] += 0 + v1n[i] * vort[i];
       eg1[i] *= -1;
@@ -634,7 +665,7 @@ static void hv_run(Oracle *o, int np1, double dt_in, double eta_ave_w, double *w
           const size_t i = (size_t)pt * L + k;
           vt0[i] *= -o->nu;
           vt1[i] *= -o->nu;
-          tt[i] *= -o->nu_s;
+          tt[i] *= PLANT(4) ? o->nu_s : -o->nu_s; /* plant 4: biharmonic sign */
           dpt[i] *= -o->nu_p;
         }
         if (o->nu_top > 0)
@@ -1030,7 +1061,7 @@ static void ppm_remap_column(int nlev, int remap_alg, const RemapGrid *g, double
   for (int j = 0; j < nlev + 1; ++j)
     ai[j] = A(j) + G[3 * W + j] * (A(j + 1) - A(j)) +
             G[4 * W + j] * (G[5 * W + j] * (G[6 * W + j] - G[7 * W + j]) * (A(j + 1) - A(j)) -
-                            G[8 * W + j] * dma[j + 1] + G[9 * W + j] * dma[j]);
+                            G[8 * W + j] * dma[PLANT(7) ? j : j + 1] + G[9 * W + j] * dma[j]); /* plant 7: index */
   for (int jp = 0; jp < nlev; ++jp) {
     const int j = jp + 1;
     double al = ai[j - 1], ar = ai[j];

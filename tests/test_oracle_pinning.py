@@ -1,0 +1,218 @@
+"""Pins the parts of the oracle that the reference's golden vectors leave
+open, with properties the mathematics fixes, and shows that each property
+catches a plausible mistake: the oracle is rebuilt with one planted mistake
+at a time (ORACLE_PLANT in oracle/preqx_oracle.c) and the matching check must
+then fail.
+
+* CAAR (the dynamics right-hand side): the unperturbed JW jet is a steady
+  solution of the primitive equations (baroclinic_inst_mod.F90:56-271 in the
+  reference); the discrete step keeps it to a truncation error that shrinks
+  with resolution.  Planted: Coriolis sign, kinetic-energy gradient dropped,
+  pressure-gradient sign.
+* Hyperviscosity: a zonal l=2 temperature harmonic Y is damped per subcycle
+  by dt nu (l(l+1)/a^2)^2 Y (explicit biharmonic step).  Planted: sign of
+  the temperature biharmonic.
+* Euler tracer step: a constant mixing ratio stays constant under the JW
+  flow (tracer fluxes consistent with the dynamics' mass fluxes; the nu_top
+  sponge acts on the dynamics only, so it is off here), and a zonally
+  symmetric mixing ratio is left unchanged by the zonal jet up to a
+  truncation error that converges at third order.  Planted: flux-divergence
+  sign, transposed derivative matrix.
+* PPM remap: a mixing ratio linear in the pressure coordinate is remapped
+  exactly in interior cells.  Planted: wrong dma index in the edge values.
+* DSS: every node's copies agree after assembly.  Planted: corner copies
+  dropped.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from hommexx_amd.problem import build_problem
+
+A = 6.376e6
+P = oracle.ptr
+
+
+# ------------------------------------------------------------- CAAR: JW jet
+def jw_residual(L, ne, nsteps=4):
+    nu = 1e15
+    pr = build_problem(ne=ne, nlev=26, qsize=1, seed=0, u_perturb=0.0, nu=nu, nu_p=nu, nu_q=nu, nu_s=nu, nu_div=nu,
+                       nu_top=0.0)
+    v0, t0, ps0 = pr.state["v"][:, 1].copy(), pr.state["T"][:, 1].copy(), pr.state["ps_v"][:, 1].copy()
+    s = oracle.OracleSession(pr, L)
+    try:
+        nstep, nm1, n0, np1 = s.run(nsteps)
+        out = s.state()
+    except RuntimeError:
+        return None
+    finally:
+        s.close()
+    du = out["v"][:, n0] - v0
+    return {"du_rms": float(np.sqrt(np.mean(du * du))), "dT_max": float(np.abs(out["T"][:, n0] - t0).max()),
+            "dps_max": float(np.abs(out["ps_v"][:, n0] - ps0).max())}
+
+
+def jw_balanced(L):
+    """The jet stays balanced: small residual at ne8 that converges from ne4."""
+    r4, r8 = jw_residual(L, 4), jw_residual(L, 8)
+    if r4 is None or r8 is None:
+        return False
+    return bool(r8["du_rms"] < 6e-3 and r8["dT_max"] < 8e-3 and r8["dps_max"] < 8.0
+                and r4["du_rms"] > 3 * r8["du_rms"])
+
+
+def test_jw_steady_jet_stays_balanced(oracle_lib):
+    assert jw_balanced(oracle_lib)
+
+
+@pytest.mark.parametrize("plant", [1, 2, 3])
+def test_jw_balance_detects_planted_dynamics_mistakes(oracle_lib, plant):
+    assert not jw_balanced(oracle.plant(plant))
+
+
+# ---------------------------------------------- hyperviscosity: Y2 damping
+def hv_damping_ratio(L, ne=4):
+    """Measured / predicted damping of a zonal l=2 temperature harmonic by the
+    hyperviscosity phase of one step (state after the phase minus state
+    before it, from two runs of the same step stopped after RK and after HV)."""
+    nu, sub, dt = 1e17, 3, 300.0
+    pr = build_problem(ne=ne, nlev=26, qsize=1, seed=0, dt=dt, nu=nu, nu_p=nu, nu_q=nu, nu_s=nu, nu_div=nu,
+                       nu_top=0.0, hypervis_subcycle=sub)
+    st = pr.state
+    x = np.sin(pr.mesh.lat)
+    y2 = 0.5 * (3 * x * x - 1)
+    st["v"][:] = 0.0
+    st["phis"][:] = 0.0
+    st["T"][:] = 280.0 + 5.0 * y2[:, None, None]
+    st["Qdp"][:] = 0.3 * st["dp3d"][:, :2, None]
+    res = []
+    for stop in (2, 3):
+        s = oracle.OracleSession(pr, L)
+        L.oracle_debug_stop(s.h, stop)
+        s.run(1)
+        res.append(s.state()["T"])
+        s.close()
+    d = res[1] - res[0]
+    lvl = int(np.argmax(np.abs(d).reshape(d.shape[0], 3, -1).max(axis=(0, 2))))
+    tb, dt_hv = res[0][:, lvl], d[:, lvl]
+    w = pr.metric["spheremp"][:, None]
+    yy = np.broadcast_to(y2[:, None], tb.shape)
+    amp = (w * (tb - tb.mean()) * yy).sum() / (w * yy * yy).sum()
+    damp = (w * dt_hv * yy).sum() / (w * yy * yy).sum()
+    lam = 6.0 / A ** 2
+    return -damp / (amp * sub * (dt / sub) * nu * lam * lam)
+
+
+def test_hyperviscosity_damps_y2_at_the_biharmonic_rate(oracle_lib):
+    r = hv_damping_ratio(oracle_lib)
+    assert abs(r - 1.0) < 0.1, r
+
+
+def test_hyperviscosity_check_detects_planted_sign(oracle_lib):
+    r = hv_damping_ratio(oracle.plant(4))
+    assert not abs(r - 1.0) < 0.1
+
+
+# ------------------------------------------ Euler step: constant mixing ratio
+def constant_q_error(L, ne=4, nsteps=3):
+    pr = build_problem(ne=ne, nlev=26, qsize=2, seed=3, nu_top=0.0)
+    pr.state["Qdp"][:] = 0.25 * pr.state["dp3d"][:, :2, None]
+    s = oracle.OracleSession(pr, L)
+    try:
+        s.run(nsteps)
+        out = s.state()
+    except RuntimeError:
+        return np.inf
+    finally:
+        s.close()
+    return float(np.abs(out["Q"][:, :2] - 0.25).max())
+
+
+def test_constant_mixing_ratio_is_kept_under_the_jet(oracle_lib):
+    assert constant_q_error(oracle_lib) < 1e-13
+
+
+def test_constant_mixing_ratio_detects_planted_flux_sign(oracle_lib):
+    assert not constant_q_error(oracle.plant(5)) < 1e-13
+
+
+def zonal_tracer_error(L, ne, nsteps=3):
+    pr = build_problem(ne=ne, nlev=26, qsize=1, seed=3, u_perturb=0.0, nu_top=0.0)
+    q0 = 0.5 + 0.3 * np.sin(pr.mesh.lat)
+    pr.state["Qdp"][:] = q0[:, None, None, None, :, :] * pr.state["dp3d"][:, :2, None]
+    s = oracle.OracleSession(pr, L)
+    try:
+        s.run(nsteps)
+        out = s.state()
+    except RuntimeError:
+        return np.inf
+    finally:
+        s.close()
+    return float(np.abs(out["Q"][:, 0] - q0[:, None]).max())
+
+
+def zonal_tracer_kept(L):
+    e4, e8 = zonal_tracer_error(L, 4), zonal_tracer_error(L, 8)
+    return bool(e4 < 1e-4 and e8 < 1e-5 and e4 > 4 * e8)
+
+
+def test_zonal_tracer_is_kept_by_the_zonal_jet(oracle_lib):
+    assert zonal_tracer_kept(oracle_lib)
+
+
+@pytest.mark.parametrize("plant", [5, 8])
+def test_zonal_tracer_detects_planted_tracer_mistakes(oracle_lib, plant):
+    assert not zonal_tracer_kept(oracle.plant(plant))
+
+
+# ------------------------------------------------------- PPM: linear profiles
+def linear_remap_error(L):
+    rng = np.random.default_rng(4)
+    n = 60
+    src = rng.uniform(100, 1000, n)
+    tgt = src * rng.uniform(0.85, 1.15, n)
+    tgt *= src.sum() / tgt.sum()
+    pio = np.concatenate([[0.0], np.cumsum(src)])
+    pin = np.concatenate([[0.0], np.cumsum(tgt)])
+    c0, c1 = 0.3, 2e-5
+    mean_src = c0 + c1 * 0.5 * (pio[1:] + pio[:-1])
+    mean_tgt = c0 + c1 * 0.5 * (pin[1:] + pin[:-1])
+    v = np.ascontiguousarray((mean_src * src)[None, :])
+    L.oracle_unit_ppm(n, 1, P(np.ascontiguousarray(src)), P(np.ascontiguousarray(tgt)), 1, P(v))
+    out = v[0] / tgt
+    return float(np.abs(out[3:-3] / mean_tgt[3:-3] - 1).max())
+
+
+def test_ppm_remaps_linear_profiles_exactly(oracle_lib):
+    assert linear_remap_error(oracle_lib) < 1e-12
+
+
+def test_ppm_linear_check_detects_planted_index(oracle_lib):
+    assert not linear_remap_error(oracle.plant(7)) < 1e-12
+
+
+# --------------------------------------------------------- DSS: continuity
+def dss_spread(L, ne=4):
+    pr = build_problem(ne=ne, nlev=26, qsize=1)
+    nl = pr.nlev
+    s = oracle.OracleSession(pr, L)
+    f = np.random.default_rng(8).standard_normal((1, pr.nelem, nl, 4, 4))
+    L.oracle_unit_dss(s.h, 1, nl, P(f), 1)
+    s.close()
+    key = np.round(pr.mesh.pos * 1e9).astype(np.int64).reshape(-1, 3)
+    _, inv = np.unique(key, axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    vals = np.transpose(f[0], (1, 0, 2, 3)).reshape(nl, -1)
+    spread = 0.0
+    for k in range(nl):
+        lo = np.full(inv.max() + 1, np.inf)
+        hi = np.full(inv.max() + 1, -np.inf)
+        np.minimum.at(lo, inv, vals[k])
+        np.maximum.at(hi, inv, vals[k])
+        spread = max(spread, float(np.max(hi - lo) / np.abs(vals[k]).max()))
+    return spread
+
+
+def test_dss_continuity_detects_planted_corner_drop(oracle_lib):
+    assert dss_spread(oracle_lib) <= 1e-13
+    assert dss_spread(oracle.plant(6)) > 1e-13
