@@ -54,10 +54,12 @@ def kernel_bytes(name, nelem, nlev, q, launches):
         s3 = s1 + q + 1
         return (s1 + s2 + s3) / 3.0 * F
     if name == "euler_dss":
-        # all: qdp_pre(Q) + fdss_pre in, Qdp(Q) + fdss out
-        # stage 2: + derived_dp, divdp_proj, dpdiss_ave in, qbih_pre(Q), qlim out; stage 3: + Qdp(n0)(Q) in
+        # qdp_pre(Q) + fdss_pre in, Qdp(Q) + fdss out; stage 3 also reads Qdp(n0)(Q) for the average
         s1 = 2 * q + 2
-        return (s1 + (s1 + 3 + q + ql) + (s1 + q)) / 3.0 * F
+        return (3 * s1 + q) / 3.0 * F
+    if name == "euler_qmm_lap":
+        # Qdp(Q), derived_dp, divdp_proj, dpdiss_ave in; qbih_pre(Q), qlim out
+        return (2 * q + 3 + ql) * F
     if name == "euler_bih_post":
         return 2 * q * F
     if name == "euler_divdp_qmm":

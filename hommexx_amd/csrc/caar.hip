@@ -2,127 +2,161 @@
 // small element-wise kernels of the step (dp3d from ps, prim_step init,
 // the (5u1-u0)/4 stage combination).
 //
-// CAAR maps one lane to one GLL column: a wave holds 4 elements x 16 points
-// and walks the column top to bottom, so the pressure, geopotential and
-// omega scans are plain serial recurrences in registers (same order as
-// CaarFunctorImpl.hpp:621-650, 689-729, 854-889), while the horizontal
-// derivatives at each level are cross-lane operations inside the 16-lane
-// group of the element.
-//   pass 1 (top->bottom): p, T_v, div(v dp), vn0 += w v dp, grad p, omega/p,
-//                         T(np1), dp3d(np1), and Rgas*T_v*dp/(2p) into phi
-//   pass 2 (bottom->top): phi = phis + 2*sum_below + own term
-//   pass 3 (top->bottom): energy gradient, vorticity, v(np1)
+// CAAR: a block holds 4 elements x 16 GLL columns x CNC level chunks; the
+// 16 lanes of one element and chunk form a slab group, so the horizontal
+// derivatives at each level are cross-lane operations inside it.  Each chunk
+// walks its own levels; the column scans stay serial recurrences in the
+// reference's order (CaarFunctorImpl.hpp:621-650, 689-729, 854-889): a chunk
+// first re-runs the recurrence over the levels above it (pressure), or reads
+// the other chunks' terms after a block barrier (omega, geopotential).
+//   A: p; vdp, vn0 += w vdp, div(vdp) -> divv scratch
+//   B: omega/p (serial sum of divv above), T(np1), dp3d(np1),
+//      Rgas*T_v*dp/(2p) -> phi
+//   C: phi = phis + 2*sum_below + own term (serial from the bottom)
+//   D: energy gradient, vorticity, v(np1)
 // Outputs go to pre-exchange buffers; the DSS kernel writes the state.
 #include "kernels.hpp"
 
 namespace hx {
 
-__global__ __launch_bounds__(64) void caar_kernel(CaarArgs a) {
-  const int lane = threadIdx.x;
-  const int pt = lane & 15;
-  const int ie = blockIdx.x * 4 + (lane >> 4);
-  if (ie >= a.nelem) return;
-  const int L = a.nlev;
-  ElemLane e;
-  load_elem(a.m, ie, pt, e);
-  DvvLane d;
-  load_dvv(a.m.dvv, pt, d);
-  const size_t sb = (size_t)ie * L * 16 + pt;  // scalar [ie][lev][pt]
-  const size_t vb = (size_t)ie * L * 32 + pt;  // vector [ie][lev][c][pt]
-  const double eta = a.eta_ave_w, dt = a.dt;
-  const double *qv = a.qdp_n0 ? a.qdp_n0 + (size_t)ie * a.qsize * L * 16 + pt : nullptr;
+constexpr int CNC = 4;  // level chunks per column
 
-  // ---------------------------------------------------------------- pass 1
-  double p_prev = a.p_top, dp_prev = 0.0, integ = 0.0;
-  for (int k = 0; k < L; ++k) {
-    const size_t s = sb + (size_t)k * 16, v = vb + (size_t)k * 32;
-    const double dp = a.dp_n0[s];
-    const double u = a.v_n0[v], w = a.v_n0[v + 16];
-    const double t = a.t_n0[s];
-    double tv = t;
-    if (qv) {
-      double qt = qv[(size_t)k * 16] / dp;
-      qt *= (RWATER_VAPOR / RGAS - 1.0);
-      qt += 1.0;
-      tv = t * qt;
-    }
-    const double p = p_prev + 0.5 * (dp_prev + dp);
-    p_prev = p;
-    dp_prev = dp;
-    const double vdp0 = u * dp, vdp1 = w * dp;
-    a.vn0[v] += eta * vdp0;
-    a.vn0[v + 16] += eta * vdp1;
-    const double divv = sph_div(e, d, vdp0, vdp1);
-    double gp0, gp1;
-    sph_grad(e, d, p, gp0, gp1);
-    const double vgrad_p = u * gp0 + w * gp1;
-    const double om = (vgrad_p - (integ + 0.5 * divv)) / p;
-    integ = integ + divv;
-    a.omega[s] += eta * om;
-    double gt0, gt1;
-    sph_grad(e, d, t, gt0, gt1);
-    const double vgrad_t = u * gt0 + w * gt1;
-    const double ttens = 0 - vgrad_t + KAPPA * tv * om;
-    double temp = ttens * dt + a.t_nm1[s];
-    temp *= e.sph;
-    a.out_t[s] = temp;
-    double tmp = 0;
-    tmp += divv;
-    tmp -= 0;
-    tmp = a.dp_nm1[s] - tmp * dt;
-    a.out_dp[s] = e.sph * tmp;
-    a.phi[s] = RGAS * tv * (dp * 0.5 / p);
+__device__ __forceinline__ double virtual_t(const CaarArgs &a, const double *qv, int k, double t, double dp) {
+  if (!qv) return t;
+  double qt = qv[(size_t)k * 16] / dp;
+  qt *= (RWATER_VAPOR / RGAS - 1.0);
+  qt += 1.0;
+  return t * qt;
+}
+
+__global__ __launch_bounds__(256) void caar_kernel(CaarArgs a) {
+  const int pt = threadIdx.x & 15;
+  const int chunk = threadIdx.x >> 6;
+  const int ie = blockIdx.x * 4 + ((threadIdx.x >> 4) & 3);
+  const bool on = ie < a.nelem;
+  const int L = a.nlev;
+  const int cs = (L + CNC - 1) / CNC;
+  const int k0 = chunk * cs < L ? chunk * cs : L;
+  const int k1 = k0 + cs < L ? k0 + cs : L;
+  ElemLane e;
+  DvvLane d;
+  if (on) {
+    load_elem(a.m, ie, pt, e);
+    load_dvv(a.m.dvv, pt, d);
   }
-  // ---------------------------------------------------------------- pass 2
+  const size_t sb = (size_t)(on ? ie : 0) * L * 16 + pt;  // scalar [ie][lev][pt]
+  const size_t vb = (size_t)(on ? ie : 0) * L * 32 + pt;  // vector [ie][lev][c][pt]
+  const double eta = a.eta_ave_w, dt = a.dt;
+  const double *qv = a.qdp_n0 ? a.qdp_n0 + (size_t)(on ? ie : 0) * a.qsize * L * 16 + pt : nullptr;
+
+  // pressure at the top of the chunk: the serial recurrence over the levels above
+  double p_top = a.p_top, dp_top = 0.0;
+  if (on)
+    for (int k = 0; k < k0; ++k) {
+      const double dp = a.dp_n0[sb + (size_t)k * 16];
+      p_top = p_top + 0.5 * (dp_top + dp);
+      dp_top = dp;
+    }
+
+  // ---------------------------------------------------------------- A
+  if (on) {
+    for (int k = k0; k < k1; ++k) {
+      const size_t s = sb + (size_t)k * 16, v = vb + (size_t)k * 32;
+      const double dp = a.dp_n0[s];
+      const double u = a.v_n0[v], w = a.v_n0[v + 16];
+      const double vdp0 = u * dp, vdp1 = w * dp;
+      a.vn0[v] += eta * vdp0;
+      a.vn0[v + 16] += eta * vdp1;
+      a.divv[s] = sph_div(e, d, vdp0, vdp1);
+    }
+  }
+  __syncthreads();
+  // ---------------------------------------------------------------- B
+  if (on) {
+    double integ = 0.0;
+    for (int k = 0; k < k0; ++k) integ = integ + a.divv[sb + (size_t)k * 16];
+    double p_prev = p_top, dp_prev = dp_top;
+    for (int k = k0; k < k1; ++k) {
+      const size_t s = sb + (size_t)k * 16, v = vb + (size_t)k * 32;
+      const double dp = a.dp_n0[s];
+      const double u = a.v_n0[v], w = a.v_n0[v + 16];
+      const double t = a.t_n0[s];
+      const double tv = virtual_t(a, qv, k, t, dp);
+      const double p = p_prev + 0.5 * (dp_prev + dp);
+      p_prev = p;
+      dp_prev = dp;
+      const double divv = a.divv[s];
+      double gp0, gp1;
+      sph_grad(e, d, p, gp0, gp1);
+      const double vgrad_p = u * gp0 + w * gp1;
+      const double om = (vgrad_p - (integ + 0.5 * divv)) / p;
+      integ = integ + divv;
+      a.omega[s] += eta * om;
+      double gt0, gt1;
+      sph_grad(e, d, t, gt0, gt1);
+      const double vgrad_t = u * gt0 + w * gt1;
+      const double ttens = 0 - vgrad_t + KAPPA * tv * om;
+      double temp = ttens * dt + a.t_nm1[s];
+      temp *= e.sph;
+      a.out_t[s] = temp;
+      double tmp = 0;
+      tmp += divv;
+      tmp -= 0;
+      tmp = a.dp_nm1[s] - tmp * dt;
+      a.out_dp[s] = e.sph * tmp;
+      a.phi[s] = RGAS * tv * (dp * 0.5 / p);
+    }
+  }
+  __syncthreads();
+  // ---------------------------------------------------------------- C
   double integration = 0.0;
-  for (int k = L - 1; k >= 0; --k) {
-    const size_t s = sb + (size_t)k * 16;
-    const double r = a.phi[s];
-    a.phi[s] = e.phis + 2.0 * integration + r;
-    integration = integration + r;
-  }
-  // ---------------------------------------------------------------- pass 3
-  p_prev = a.p_top;
-  dp_prev = 0.0;
-  for (int k = 0; k < L; ++k) {
-    const size_t s = sb + (size_t)k * 16, v = vb + (size_t)k * 32;
-    const double dp = a.dp_n0[s];
-    const double u = a.v_n0[v], w = a.v_n0[v + 16];
-    const double t = a.t_n0[s];
-    double tv = t;
-    if (qv) {
-      double qt = qv[(size_t)k * 16] / dp;
-      qt *= (RWATER_VAPOR / RGAS - 1.0);
-      qt += 1.0;
-      tv = t * qt;
+  if (on)
+    for (int k = L - 1; k >= k1; --k) integration = integration + a.phi[sb + (size_t)k * 16];
+  __syncthreads();  // every chunk has read the terms below it before they are overwritten
+  if (on)
+    for (int k = k1 - 1; k >= k0; --k) {
+      const size_t s = sb + (size_t)k * 16;
+      const double r = a.phi[s];
+      a.phi[s] = e.phis + 2.0 * integration + r;
+      integration = integration + r;
     }
-    const double p = p_prev + 0.5 * (dp_prev + dp);
-    p_prev = p;
-    dp_prev = dp;
-    double gp0, gp1;
-    sph_grad(e, d, p, gp0, gp1);
-    double eg0 = RGAS * (tv / p) * gp0;
-    double eg1 = RGAS * (tv / p) * gp1;
-    const double ephi = 0.5 * (u * u + w * w) + a.phi[s];
-    sph_grad_update(e, d, ephi, eg0, eg1);
-    double vort = sph_vort(e, d, u, w);
-    vort += e.fcor;
-    eg0 *= -1;
-    eg0 += 0 + w * vort;
-    eg1 *= -1;
-    eg1 += 0 - u * vort;
-    eg0 *= dt;
-    eg0 += a.v_nm1[v];
-    eg1 *= dt;
-    eg1 += a.v_nm1[v + 16];
-    a.out_v[v] = e.sph * eg0;
-    a.out_v[v + 16] = e.sph * eg1;
+  // ---------------------------------------------------------------- D
+  if (on) {
+    double p_prev = p_top, dp_prev = dp_top;
+    for (int k = k0; k < k1; ++k) {
+      const size_t s = sb + (size_t)k * 16, v = vb + (size_t)k * 32;
+      const double dp = a.dp_n0[s];
+      const double u = a.v_n0[v], w = a.v_n0[v + 16];
+      const double t = a.t_n0[s];
+      const double tv = virtual_t(a, qv, k, t, dp);
+      const double p = p_prev + 0.5 * (dp_prev + dp);
+      p_prev = p;
+      dp_prev = dp;
+      double gp0, gp1;
+      sph_grad(e, d, p, gp0, gp1);
+      double eg0 = RGAS * (tv / p) * gp0;
+      double eg1 = RGAS * (tv / p) * gp1;
+      const double ephi = 0.5 * (u * u + w * w) + a.phi[s];
+      sph_grad_update(e, d, ephi, eg0, eg1);
+      double vort = sph_vort(e, d, u, w);
+      vort += e.fcor;
+      eg0 *= -1;
+      eg0 += 0 + w * vort;
+      eg1 *= -1;
+      eg1 += 0 - u * vort;
+      eg0 *= dt;
+      eg0 += a.v_nm1[v];
+      eg1 *= dt;
+      eg1 += a.v_nm1[v + 16];
+      a.out_v[v] = e.sph * eg0;
+      a.out_v[v + 16] = e.sph * eg1;
+    }
   }
 }
 
 void launch_caar(hipStream_t st, const CaarArgs &a) {
   const int blocks = (a.nelem + 3) / 4;
-  hipLaunchKernelGGL(caar_kernel, dim3(blocks), dim3(64), 0, st, a);
+  hipLaunchKernelGGL(caar_kernel, dim3(blocks), dim3(64 * CNC), 0, st, a);
 }
 
 // ------------------------------------------------------------------ small

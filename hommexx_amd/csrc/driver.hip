@@ -78,7 +78,7 @@ struct Ctx {
   // forcing
   double *ft = nullptr, *fm = nullptr, *fq = nullptr;
   // scratch
-  double *cv = nullptr, *ct = nullptr, *cdp = nullptr;
+  double *cv = nullptr, *ct = nullptr, *cdp = nullptr, *cdiv = nullptr;
   double *vt_a = nullptr, *tt_a = nullptr, *dpt_a = nullptr, *vt_b = nullptr, *tt_b = nullptr, *dpt_b = nullptr;
   double *qdp_pre = nullptr, *qbih = nullptr, *qbih_pre = nullptr, *qlim[2] = {}, *fdss_pre = nullptr;
   int *err_d = nullptr;
@@ -284,6 +284,7 @@ void alloc_work() {
   c.cv = dalloc<double>(2 * e3);
   c.ct = dalloc<double>(e3);
   c.cdp = dalloc<double>(e3);
+  c.cdiv = dalloc<double>(e3);
   c.vt_a = dalloc<double>(2 * e3);
   c.vt_b = dalloc<double>(2 * e3);
   c.tt_a = dalloc<double>(e3);
@@ -319,6 +320,7 @@ void caar_stage(int nm1, int n0, int np1, double dt, double eta_ave_w) {
   a.vn0 = c.vn0;
   a.omega = c.omega;
   a.phi = c.phi;
+  a.divv = c.cdiv;
   a.out_v = c.cv;
   a.out_t = c.ct;
   a.out_dp = c.cdp;

@@ -41,15 +41,6 @@ __device__ __forceinline__ void load_elem_light(const MetricPtrs &m, int ie, int
   e.rsmp = m.rspheremp[s];
 }
 
-__device__ __forceinline__ double gather(const double *in, size_t es, size_t off, int ie, int pt, int s0, int s1,
-                                         int s2) {
-  double x = in[(size_t)ie * es + off + pt];
-  if (s0 >= 0) x += in[(size_t)(s0 >> 4) * es + off + (s0 & 15)];
-  if (s1 >= 0) x += in[(size_t)(s1 >> 4) * es + off + (s1 & 15)];
-  if (s2 >= 0) x += in[(size_t)(s2 >> 4) * es + off + (s2 & 15)];
-  return x;
-}
-
 // --------------------------------------------------------------------------
 // divdp = div(vn0), divdp_proj = divdp (EulerStepFunctorImpl.hpp:348-377) and
 // the stage-1 compute_dp + compute_qmin_qmax (:406-485)
@@ -179,7 +170,7 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
     const double diss_fac = -a.rhs_viss * a.dt * a.nu_q;
     dpdissk += diss_fac * a.dpdiss_bih[s] / e.sph;
   }
-  a.fdss_pre[s] = a.fdss[s] * e.sph;
+  a.fdss_pre[((size_t)ie * 16 + pt) * L + k] = a.fdss[s] * e.sph;  // [ie][pt][lev]
   const bool add_hv = a.rhs_viss != 0.0;
   const bool limit = a.limiter_option == 8 || a.limiter_option == 9;
   const double c = e.sph * dpdissk;
@@ -222,43 +213,93 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
       if (wmin) ql[k] = minp;
       if (wmax) ql[L + k] = maxp;
     }
-    a.qdp_pre[qi] = e.sph * ptens;
+    a.qdp_pre[(((size_t)ie * Q + q) * 16 + pt) * L + k] = e.sph * ptens;  // [ie][q][pt][lev]
   }
 }
 
-// DSS of tracers [QB*blockIdx.y, QB*blockIdx.y + QB) and, in the first
-// tracer batch, of the extra field.  avg: stage 3's SSP average
-// (Qdp(n0) + 2 Qdp(np1)) / 3 (:379-403) of the assembled tracer value.
-// All gathers of a lane are issued before its stores.
-constexpr int QB = 8;
+// ---------------------------------------------------------------- exchanges
+// The pre-exchange buffers (qdp_pre, qbih_pre, fdss_pre) are stored levels
+// fastest, [ie][q][pt][lev], so a neighbour copy of 16 consecutive levels of
+// one GLL point is one contiguous 128-byte run.  An exchange block is one
+// element x 16 levels (XL): lane (pt, kk) gathers point pt at level k0+kk,
+// then the assembled values go through an LDS tile to lanes (kk, pt) and are
+// stored in the slab layout [ie][q][lev][pt].  Blocks walk the elements of
+// one level group in order, so neighbour runs come from L2.
+constexpr int XL = 16;
+constexpr int QB = 8;  // tracers per exchange block
+
+inline int xchg_blocks_x(int nelem, int nlev) {
+  const int nb = nelem * ((nlev + XL - 1) / XL);
+  return (nb + 7) & ~7;
+}
+
+struct XLane {
+  int ie, k0, pt, kk, k;  // gather role: point pt, level k = k0 + kk
+  int wk, wpt;            // store role: level k0 + wk, point wpt
+};
+
+__device__ __forceinline__ bool xchg_lane(int nelem, int nlev, XLane &x) {
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  x.ie = lb % nelem;
+  const int g = lb / nelem;
+  x.k0 = g * XL;
+  if (x.k0 >= nlev) return false;  // whole block
+  x.pt = threadIdx.x >> 4;
+  x.kk = threadIdx.x & 15;
+  x.k = x.k0 + x.kk;
+  x.wk = threadIdx.x >> 4;
+  x.wpt = threadIdx.x & 15;
+  return true;
+}
+
+// gather of point (ie, pt) at level k from a levels-fastest field with
+// element stride es (own value, then the sources in the reference's order)
+__device__ __forceinline__ double gather_lf(const double *f, size_t es, size_t off, int L, int ie, int pt, int k,
+                                            int s0, int s1, int s2) {
+  double x = f[(size_t)ie * es + off + (size_t)pt * L + k];
+  if (s0 >= 0) x += f[(size_t)(s0 >> 4) * es + off + (size_t)(s0 & 15) * L + k];
+  if (s1 >= 0) x += f[(size_t)(s1 >> 4) * es + off + (size_t)(s1 & 15) * L + k];
+  if (s2 >= 0) x += f[(size_t)(s2 >> 4) * es + off + (size_t)(s2 & 15) * L + k];
+  return x;
+}
+
+// DSS of tracers [QB*blockIdx.y, QB*blockIdx.y + QB) and, in the first batch,
+// of the extra field.  avg: stage 3's SSP average (Qdp(n0) + 2 Qdp(np1)) / 3
+// (:379-403) of the assembled tracer value.
 __global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int avg) {
-  int ie, k, pt;
-  if (!group_slab(a.nelem, a.nlev, ie, k, pt)) return;
+  __shared__ double tile[2][XL][17];
+  XLane x;
+  if (!xchg_lane(a.nelem, a.nlev, x)) return;
   const int L = a.nlev, Q = a.qsize, q0 = blockIdx.y * QB;
-  const int *src = a.src + ((size_t)ie * 16 + pt) * 3;
+  const int *src = a.src + ((size_t)x.ie * 16 + x.pt) * 3;
   const int s0 = src[0], s1 = src[1], s2 = src[2];
-  const double r = a.m.rspheremp[(size_t)ie * 16 + pt];
-  const size_t es = (size_t)Q * L * 16, e3 = (size_t)L * 16, so = (size_t)k * 16;
-  double x[QB], avg0[QB], xf = 0;
+  const double r = a.m.rspheremp[(size_t)x.ie * 16 + x.pt];
+  const bool gk = x.k < L, wkv = x.k0 + x.wk < L;
+  const size_t es = (size_t)Q * 16 * L;
+  const int nq = (Q - q0) < QB ? (Q - q0) : QB;
+  double v[QB];
+#pragma unroll
+  for (int i = 0; i < QB; ++i)
+    if (i < nq && gk) v[i] = gather_lf(a.qdp_pre, es, (size_t)(q0 + i) * 16 * L, L, x.ie, x.pt, x.k, s0, s1, s2);
 #pragma unroll
   for (int i = 0; i < QB; ++i) {
-    if (q0 + i < Q) {
-      const size_t off = ((size_t)(q0 + i) * L + k) * 16;
-      x[i] = gather(a.qdp_pre, es, off, ie, pt, s0, s1, s2);
-      avg0[i] = avg ? a.qdp_avg_n0[(size_t)ie * es + off + pt] : 0.0;
+    if (i >= nq) break;
+    double (*t)[17] = tile[i & 1];
+    if (gk) t[x.kk][x.pt] = v[i] * r;
+    __syncthreads();
+    if (wkv) {
+      const size_t o = (((size_t)x.ie * Q + q0 + i) * L + x.k0 + x.wk) * 16 + x.wpt;
+      double y = t[x.wk][x.wpt];
+      if (avg) y = (a.qdp_avg_n0[o] + (3.0 - 1) * y) / 3.0;
+      a.qdp_np1[o] = y;
     }
   }
-  if (q0 == 0) xf = gather(a.fdss_pre, e3, so, ie, pt, s0, s1, s2);
-#pragma unroll
-  for (int i = 0; i < QB; ++i) {
-    if (q0 + i < Q) {
-      const size_t off = ((size_t)(q0 + i) * L + k) * 16;
-      double y = x[i] * r;
-      if (avg) y = (avg0[i] + (3.0 - 1) * y) / 3.0;
-      a.qdp_np1[(size_t)ie * es + off + pt] = y;
-    }
+  if (q0 == 0) {
+    __syncthreads();
+    if (gk) tile[0][x.kk][x.pt] = gather_lf(a.fdss_pre, (size_t)16 * L, 0, L, x.ie, x.pt, x.k, s0, s1, s2) * r;
+    __syncthreads();
+    if (wkv) a.fdss[((size_t)x.ie * L + x.k0 + x.wk) * 16 + x.wpt] = tile[0][x.wk][x.wpt];
   }
-  if (q0 == 0) a.fdss[(size_t)ie * e3 + so + pt] = xf * r;
 }
 
 // Stage 3 compute_dp + compute_qmin_qmax (rhs_multiplier 2, :406-485) and the
@@ -289,36 +330,46 @@ __global__ __launch_bounds__(256) void euler_qmm_lap_kernel(EulerArgs a) {
     }
     double y = x;
     if (a.nu_p > 0) y = y * wdp / dp0;
-    a.qbih_pre[qi] = sph_laplace(e, d, y);
+    a.qbih_pre[(((size_t)ie * Q + q) * 16 + pt) * L + k] = sph_laplace(e, d, y);  // [ie][q][pt][lev]
   }
 }
 
 // biharmonic, second half (:216-310): DSS of the first laplacian (with
 // rspheremp), second laplacian, times -rhs_viss*dt*nu_q*dp0/spheremp, for
-// tracers [QB*blockIdx.y, QB*blockIdx.y + QB).
+// tracers [QB*blockIdx.y, QB*blockIdx.y + QB).  After the LDS transpose the
+// 16-lane groups are slabs (one level, 16 points) for the laplacian.
 __global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
-  int ie, k, pt;
-  if (!group_slab(a.nelem, a.nlev, ie, k, pt)) return;
+  __shared__ double tile[2][XL][17];
+  XLane x;
+  if (!xchg_lane(a.nelem, a.nlev, x)) return;
   const int L = a.nlev, Q = a.qsize, q0 = blockIdx.y * QB;
-  const int *src = a.src + ((size_t)ie * 16 + pt) * 3;
+  const int *src = a.src + ((size_t)x.ie * 16 + x.pt) * 3;
   const int s0 = src[0], s1 = src[1], s2 = src[2];
-  const size_t es = (size_t)Q * L * 16;
-  double x[QB];
+  const bool gk = x.k < L;
+  const int wk = x.k0 + x.wk;
+  const bool wkv = wk < L;
+  const size_t es = (size_t)Q * 16 * L;
+  const int nq = (Q - q0) < QB ? (Q - q0) : QB;
+  double v[QB];
 #pragma unroll
   for (int i = 0; i < QB; ++i)
-    if (q0 + i < Q) x[i] = gather(a.qbih_pre, es, ((size_t)(q0 + i) * L + k) * 16, ie, pt, s0, s1, s2);
+    if (i < nq && gk) v[i] = gather_lf(a.qbih_pre, es, (size_t)(q0 + i) * 16 * L, L, x.ie, x.pt, x.k, s0, s1, s2);
   ElemLane e;
-  load_elem_light(a.m, ie, pt, e);
+  load_elem_light(a.m, x.ie, x.wpt, e);
   DvvLane d;
-  load_dvv(a.m.dvv, pt, d);
+  load_dvv(a.m.dvv, x.wpt, d);
   const double f = -a.rhs_viss * a.dt * a.nu_q;
-  const double dp0 = a.dp0[k];
+  const double dp0 = wkv ? a.dp0[wk] : 0.0;
 #pragma unroll
   for (int i = 0; i < QB; ++i) {
-    if (q0 + i < Q) {
-      const double lap = sph_laplace(e, d, x[i] * e.rsmp);
-      a.qbih[(size_t)ie * es + ((size_t)(q0 + i) * L + k) * 16 + pt] = (f * dp0 * lap / e.sph);
-    }
+    if (i >= nq) break;
+    double (*t)[17] = tile[i & 1];
+    if (gk) t[x.kk][x.pt] = v[i];
+    __syncthreads();
+    // every lane of a slab takes part in the laplacian's cross-lane reads
+    const double y = wkv ? t[x.wk][x.wpt] * e.rsmp : 0.0;
+    const double lap = sph_laplace(e, d, y);
+    if (wkv) a.qbih[(((size_t)x.ie * Q + q0 + i) * L + wk) * 16 + x.wpt] = (f * dp0 * lap / e.sph);
   }
 }
 
@@ -339,7 +390,7 @@ void launch_euler_advect(hipStream_t st, const EulerArgs &a, int stage) {
 
 void launch_euler_dss(hipStream_t st, const EulerArgs &a, int avg) {
   const int nq = a.qsize > 0 ? (a.qsize + QB - 1) / QB : 1;
-  hipLaunchKernelGGL(euler_dss_kernel, dim3(group_blocks_x(a.nelem, a.nlev), nq), dim3(256), 0, st, a, avg);
+  hipLaunchKernelGGL(euler_dss_kernel, dim3(xchg_blocks_x(a.nelem, a.nlev), nq), dim3(256), 0, st, a, avg);
 }
 
 void launch_euler_qmm_lap(hipStream_t st, const EulerArgs &a) {
@@ -348,7 +399,7 @@ void launch_euler_qmm_lap(hipStream_t st, const EulerArgs &a) {
 
 void launch_euler_bih_post(hipStream_t st, const EulerArgs &a) {
   if (a.qsize <= 0) return;
-  hipLaunchKernelGGL(euler_bih_post_kernel, dim3(group_blocks_x(a.nelem, a.nlev), (a.qsize + QB - 1) / QB), dim3(256),
+  hipLaunchKernelGGL(euler_bih_post_kernel, dim3(xchg_blocks_x(a.nelem, a.nlev), (a.qsize + QB - 1) / QB), dim3(256),
                      0, st, a);
 }
 

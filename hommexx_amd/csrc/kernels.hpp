@@ -17,6 +17,7 @@ struct CaarArgs {
   const double *v_n0, *t_n0, *dp_n0, *v_nm1, *t_nm1, *dp_nm1;
   const double *qdp_n0;  // moist: tracer block of the n0_qdp level (tracer 0 used), else nullptr
   double *vn0, *omega, *phi;
+  double *divv;                 // scratch: div(v dp) per level
   double *out_v, *out_t, *out_dp;
   double dt, eta_ave_w, p_top;  // p_top = hyai(0)*ps0
 };

@@ -125,6 +125,14 @@ class HipSession:
         L.init_boundary_exchanges_c()
         L.init_time_level_c(*[_r(x, _I) for x in prob.time_level])
         self._check("init")
+        if getattr(prob, "forcing", None) is not None:
+            # prim_driver_mod.F90 pushes FM, FT, FQ (and Qdp) before the step
+            f = prob.forcing
+            fa = [f64(f["FM"]), f64(f["FT"]), f64(f["FQ"]), st[3]]
+            self._keep += fa
+            P = ctypes.POINTER(_D)
+            L.f90_push_forcing_to_cxx(*[a.ctypes.data_as(P) for a in fa])
+            self._check("f90_push_forcing_to_cxx")
         self.tl = None
 
     def _check(self, what):

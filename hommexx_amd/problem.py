@@ -56,6 +56,9 @@ class Problem:
     state: dict = field(repr=False)
     connections: list = field(repr=False)
     time_level: tuple = (1, 2, 3, 0, 0)  # nm1, n0, np1 (1-based), nstep, nstep0
+    # CAM forcing (ftype 0/2) as f90_push_forcing_to_cxx receives it:
+    # {"FM": (nelem,nlev,2,4,4), "FT": (nelem,nlev,4,4), "FQ": (nelem,qsize_d,nlev,4,4)}
+    forcing: dict = field(default=None, repr=False)
 
     @property
     def nelem(self):
@@ -86,3 +89,24 @@ def build_problem(ne=4, nlev=26, qsize=4, seed=0, dt=300.0, t_noise=0.0, u_pertu
     return Problem(ne=ne, nlev=nlev, qsize=qsize, dt=dt, params=p, mesh=mesh, vcoord=vc,
                    dvv=np.ascontiguousarray(derivative_matrix()), metric=mesh.fortran_arrays(),
                    state=state, connections=list(mesh.connections))
+
+
+def make_forcing(prob, seed=0, ft_scale=1e-4, fm_scale=1e-5, fq_scale=1e-7):
+    """Seeded smooth CAM-style forcing tendencies for ``prob`` (K/s, m/s^2,
+    kg/m^2/s per level): a solid-body-rotation FM, an l<=2 pattern in FT and
+    per-tracer patterns in FQ whose negative parts exercise the clip."""
+    rng = np.random.default_rng(seed)
+    m = prob.mesh
+    x, y, z = m.pos[..., 0], m.pos[..., 1], m.pos[..., 2]
+    lat = m.lat
+    nl = prob.nlev
+    kk = np.linspace(0.5, 1.0, nl)[None, :, None, None]
+    FT = ft_scale * (z + 0.5 * x * y)[:, None] * kk
+    FM = np.zeros((prob.nelem, nl, 2, 4, 4))
+    FM[:, :, 0] = fm_scale * np.cos(lat)[:, None] * kk
+    FM[:, :, 1] = 0.3 * fm_scale * (x * z)[:, None] * kk
+    FQ = np.zeros((prob.nelem, prob.qsize_d, nl, 4, 4))
+    for q in range(prob.qsize):
+        c = rng.standard_normal(4)
+        FQ[:, q] = fq_scale * (c[0] + c[1] * x + c[2] * y + c[3] * z)[:, None] * kk
+    return {"FM": np.ascontiguousarray(FM), "FT": np.ascontiguousarray(FT), "FQ": np.ascontiguousarray(FQ)}

@@ -54,6 +54,7 @@ def _bind(path):
     L.oracle_unit_dss.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int]
     L.oracle_num_threads.restype = ctypes.c_int
     L.oracle_debug_stop.argtypes = [_vp, ctypes.c_int]
+    L.oracle_set_forcing.argtypes = [_vp, _dp, _dp, _dp]
     return L
 
 
@@ -119,6 +120,10 @@ class OracleSession:
         L.oracle_set_states(self.h, ptr(f64(s["v"])), ptr(f64(s["T"])), ptr(f64(s["dp3d"])), ptr(f64(s["Qdp"])),
                             ptr(f64(s["ps_v"])))
         L.oracle_set_time_level(self.h, *prob.time_level)
+        if getattr(prob, "forcing", None) is not None:
+            f = prob.forcing
+            self._keep["forcing"] = [f64(f["FM"]), f64(f["FT"]), f64(f["FQ"])]
+            L.oracle_set_forcing(self.h, *[ptr(a) for a in self._keep["forcing"]])
         self.tl = None
 
     def run(self, nsteps=1, dt=None):

@@ -35,6 +35,9 @@
  *                     mass and 1-norm properties (preqx_ut.cpp:1335-1532)
  *   PPM remap         identity, constants, column mass, monotone range,
  *                     linear profiles exact in the interior
+ *   CAM forcing       forced tracer and (moist) surface-pressure mass equal the
+ *                     initial mass plus the clipped dt*FQ; a uniform FT raises
+ *                     T by dt*FT
  *   full step         dry and tracer mass conserved to 1e-14
  * Each check is shown to fail on a build with one planted mistake
  * (ORACLE_PLANT below).  The exact accumulation order of the DSS follows
@@ -95,6 +98,7 @@ typedef struct {
   double *phi, *omega_p, *eta_dot_dpdn, *dpdiss_ave, *dpdiss_bih, *derived_dp, *divdp, *divdp_proj, *vn0;
   /* buffers */
   double *qtens_bih, *qlim, *vstar, *dpdissk, *dpbuf, *ttens, *dptens, *vtens;
+  double *fm, *ft, *fq; /* CAM forcing, [ie][c][pt][lev], [ie][pt][lev], [ie][q][pt][lev] */
   double *dss_buf;   /* [nelem][8][NP][max fields * nlev] */
   int dss_buf_fields;
   /* time levels */
@@ -1208,6 +1212,7 @@ Oracle *oracle_create(int nelem, int nlev, int qsize, int qsize_d) {
   o->dpdiss_ave = alloc0(ne * e3); o->dpdiss_bih = alloc0(ne * e3); o->derived_dp = alloc0(ne * e3);
   o->divdp = alloc0(ne * e3); o->divdp_proj = alloc0(ne * e3); o->vn0 = alloc0(ne * 2 * e3);
   o->qtens_bih = alloc0(ne * (qsize ? qsize : 1) * e3);
+  o->fm = alloc0(ne * 2 * e3); o->ft = alloc0(ne * e3); o->fq = alloc0(ne * (qsize ? qsize : 1) * e3);
   o->qlim = alloc0(ne * (qsize ? qsize : 1) * 2 * nlev);
   o->vstar = alloc0(ne * 2 * e3); o->dpdissk = alloc0(ne * e3); o->dpbuf = alloc0(ne * e3);
   o->ttens = alloc0(ne * e3); o->dptens = alloc0(ne * e3); o->vtens = alloc0(ne * 2 * e3);
@@ -1221,7 +1226,7 @@ void oracle_destroy(Oracle *o) {
                     o->rspheremp, o->metdet, o->phis, o->hyai, o->hybi, o->hyai_delta, o->hybi_delta, o->dp0,
                     o->v, o->t, o->dp3d, o->ps_v, o->qdp, o->Q, o->phi, o->omega_p, o->eta_dot_dpdn,
                     o->dpdiss_ave, o->dpdiss_bih, o->derived_dp, o->divdp, o->divdp_proj, o->vn0, o->qtens_bih,
-                    o->qlim, o->vstar, o->dpdissk, o->dpbuf, o->ttens, o->dptens, o->vtens, o->dss_buf};
+                    o->qlim, o->vstar, o->dpdissk, o->dpbuf, o->ttens, o->dptens, o->vtens, o->dss_buf, o->fm, o->ft, o->fq};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
   free(o->conn_lid); free(o->conn_pos); free(o->conn_dir);
   free(o);
@@ -1234,7 +1239,7 @@ int oracle_set_params(Oracle *o, int remap_alg, int limiter_option, int rsplit, 
                       double hypervis_scaling, int ftype, int moist) {
   if (remap_alg < 1 || remap_alg > 3 || (limiter_option != 8 && limiter_option != 9) || time_step_type != 5 ||
       hypervis_order != 2 || rsplit < 1 || qsplit < 1 || !(nu > 0) || !(nu_p > 0) || !(nu_div > 0) ||
-      qsize != o->qsize || ftype != -1)
+      qsize != o->qsize || (ftype != -1 && ftype != 0 && ftype != 2))
     return 1;
   if (hypervis_scaling != 0.0) return 4; /* tensor hyperviscosity is not restated */
   o->remap_alg = remap_alg; o->limiter_option = limiter_option; o->rsplit = rsplit; o->qsplit = qsplit;
@@ -1333,6 +1338,67 @@ void oracle_set_states(Oracle *o, const double *v, const double *t, const double
   }
 }
 
+/* f90_push_forcing_to_cxx (cxx_f90_interface.cpp:180-205): FM (np,np,2,nlev),
+   FT (np,np,nlev), FQ (np,np,nlev,qsize_d) per element, Fortran order */
+void oracle_set_forcing(Oracle *o, const double *fm, const double *ft, const double *fq) {
+  const size_t e3 = E3;
+  for (int ie = 0; ie < o->nelem; ++ie) {
+    for (int c = 0; c < 2; ++c)
+      for (int k = 0; k < L; ++k)
+        for (int p = 0; p < NP2; ++p)
+          o->fm[((size_t)ie * 2 + c) * e3 + p * L + k] = fm[(((size_t)ie * L + k) * 2 + c) * NP2 + p];
+    f2o3(ft + (size_t)ie * e3, o->ft + (size_t)ie * e3, L);
+    if (fq)
+      for (int q = 0; q < o->qsize; ++q)
+        f2o3(fq + ((size_t)ie * o->qsize_d + q) * e3, o->fq + ((size_t)ie * o->qsize + q) * e3, L);
+  }
+}
+
+/* apply_cam_forcing / apply_cam_forcing_dynamics (CamForcing.cpp:20-176):
+   T, v at n0 += dt F; for ftype 0 also the tracers at n0_qdp, with the
+   negative-mass clip, the moist surface-pressure update from tracer 0
+   (summed over levels in order) and Q = Qdp / dp(ps_v). */
+static void cam_forcing(Oracle *o, double dt, int tracers) {
+  const size_t e3 = E3;
+#pragma omp parallel for schedule(static)
+  for (int ie = 0; ie < o->nelem; ++ie) {
+    double *t = T(ie, o->n0), *v0 = V(ie, o->n0, 0), *v1 = V(ie, o->n0, 1);
+    const double *ft = o->ft + (size_t)ie * e3, *fm0 = o->fm + (size_t)ie * 2 * e3, *fm1 = fm0 + e3;
+    for (size_t i = 0; i < e3; ++i) t[i] += (PLANT(9) ? -dt : dt) * ft[i]; /* plant 9: forcing sign */
+    for (size_t i = 0; i < e3; ++i) v0[i] += dt * fm0[i];
+    for (size_t i = 0; i < e3; ++i) v1[i] += dt * fm1[i];
+    if (!tracers) continue;
+    if (o->moist && o->qsize > 0) {
+      const double *fq0 = o->fq + (size_t)ie * o->qsize * e3, *q0 = QDP(ie, o->n0_qdp, 0);
+      for (int p = 0; p < NP2; ++p) {
+        double acc = 0.0;
+        for (int k = 0; k < L; ++k) {
+          double v = dt * fq0[p * L + k];
+          const double qs = q0[p * L + k];
+          if (qs + v < 0.0 && v < 0.0) v = qs < 0.0 ? 0.0 : -qs;
+          acc += v;
+        }
+        PSV(ie, o->n0)[p] += acc;
+      }
+    }
+    for (int q = 0; q < o->qsize; ++q) {
+      double *qd = QDP(ie, o->n0_qdp, q);
+      const double *fq = o->fq + ((size_t)ie * o->qsize + q) * e3;
+      for (size_t i = 0; i < e3; ++i) {
+        double v = dt * fq[i];
+        if (!PLANT(10) && qd[i] + v < 0.0 && v < 0.0) v = qd[i] < 0.0 ? 0.0 : -qd[i]; /* plant 10: no clip */
+        qd[i] += v;
+      }
+    }
+    for (int q = 0; q < o->qsize; ++q)
+      for (int p = 0; p < NP2; ++p)
+        for (int k = 0; k < L; ++k) {
+          const double dp = o->hyai_delta[k] * o->ps0 + o->hybi_delta[k] * PSV(ie, o->n0)[p];
+          o->Q[((size_t)ie * o->qsize + q) * e3 + p * L + k] = QDP(ie, o->n0_qdp, q)[p * L + k] / dp;
+        }
+  }
+}
+
 void oracle_get_states(Oracle *o, double *v, double *t, double *dp3d, double *qdp, double *Q, double *ps_v,
                        double *omega_p) {
   const size_t e3 = E3;
@@ -1372,7 +1438,8 @@ int oracle_run_subcycle(Oracle *o, double dt, int *nstep, int *nm1, int *n0, int
   }
   const double dt_q = dt * o->qsplit;
   const double dt_remap = dt_q * o->rsplit;
-  (void)dt_remap;
+  if (o->ftype == 0) cam_forcing(o, dt_remap, 1);
+  else if (o->ftype == 2) cam_forcing(o, dt_remap, 0);
   /* dp3d from ps */
 #pragma omp parallel for schedule(static)
   for (int ie = 0; ie < o->nelem; ++ie)

@@ -10,7 +10,7 @@ import pytest
 
 import oracle
 from hommexx_amd.hip import HipSession, HommexxError
-from hommexx_amd.problem import build_problem
+from hommexx_amd.problem import build_problem, make_forcing
 from parity import compare, fmt
 
 pytestmark = pytest.mark.gpu
@@ -84,6 +84,18 @@ def test_option_parity(over):
     print(over, fmt(rep))
     for k, (e, _) in rep.items():
         assert e <= TOL1 * 10, (over, k, e)
+
+
+@pytest.mark.parametrize("ftype,moist", [(0, False), (0, True), (2, False)])
+def test_forcing_parity(ftype, moist):
+    """CAM forcing (CamForcing.cpp), pushed with f90_push_forcing_to_cxx."""
+    prob = build_problem(ne=4, nlev=26, qsize=3, seed=7, ftype=ftype, moisture=moist)
+    prob.forcing = make_forcing(prob, seed=7, fq_scale=3e-5)
+    out, ref = run_both(prob, 2)
+    rep = compare(out, ref)
+    print(ftype, moist, fmt(rep))
+    for k, (e, _) in rep.items():
+        assert e <= TOL1 * 10, (ftype, moist, k, e)
 
 
 def test_l72_parity():

@@ -22,12 +22,16 @@ then fail.
   exactly in interior cells.  Planted: wrong dma index in the edge values.
 * DSS: every node's copies agree after assembly.  Planted: corner copies
   dropped.
+* CAM forcing (CamForcing.cpp): after a forced step the tracer mass (and,
+  moist, the surface-pressure mass) equals the initial mass plus the clipped
+  dt*FQ, since the step itself conserves mass; a uniform FT raises T by
+  dt*FT.  Planted: forcing sign, missing negative-mass clip.
 """
 import numpy as np
 import pytest
 
 import oracle
-from hommexx_amd.problem import build_problem
+from hommexx_amd.problem import build_problem, make_forcing
 
 A = 6.376e6
 P = oracle.ptr
@@ -216,3 +220,65 @@ def dss_spread(L, ne=4):
 def test_dss_continuity_detects_planted_corner_drop(oracle_lib):
     assert dss_spread(oracle_lib) <= 1e-13
     assert dss_spread(oracle.plant(6)) > 1e-13
+
+
+# ---------------------------------------------------------------- forcing
+def forced_mass_errors(L, moist, fq_scale):
+    pr = build_problem(ne=4, nlev=26, qsize=3, seed=7, ftype=0, moisture=moist)
+    pr.forcing = make_forcing(pr, 7, fq_scale=fq_scale)
+    sph = pr.metric["spheremp"]
+    q0 = pr.state["Qdp"][:, 0, :3]
+    v = pr.dt * pr.forcing["FQ"][:, :3]
+    clip = np.where((q0 + v < 0) & (v < 0), np.where(q0 < 0, 0.0, -q0), v)
+    s = oracle.OracleSession(pr, L)
+    nstep, nm1, n0, np1 = s.run(1)
+    out = s.state()
+    s.close()
+    want = np.einsum("epq,eclpq->c", sph, q0 + clip)
+    got = np.einsum("epq,eclpq->c", sph, out["Qdp"][:, nstep % 2, :3])
+    eq = float(np.max(np.abs(got / want - 1)))
+    ps0 = (sph * pr.state["ps_v"][:, 1]).sum()
+    dps = np.einsum("epq,elpq->", sph, clip[:, 0]) if moist else 0.0
+    eps = abs((sph * out["ps_v"][:, n0]).sum() / (ps0 + dps) - 1)
+    return eq, eps, float(np.mean(clip != v))
+
+
+@pytest.mark.parametrize("moist,scale", [(False, 1.0), (True, 0.2)])
+def test_forced_mass_is_initial_plus_clipped_forcing(oracle_lib, moist, scale):
+    eq, eps, clipped = forced_mass_errors(oracle_lib, moist, scale)
+    assert clipped > 0  # the clip is exercised
+    assert eq < 1e-13 and eps < 1e-14
+
+
+def test_forced_mass_detects_planted_missing_clip(oracle_lib):
+    eq, _, _ = forced_mass_errors(oracle.plant(10), False, 1.0)
+    assert not eq < 1e-13
+
+
+def uniform_ft_response(L, c=1e-3):
+    def run(ftype):
+        pr = build_problem(ne=4, nlev=26, qsize=1, seed=7, ftype=ftype)
+        f = make_forcing(pr, 0)
+        f["FT"][:] = c
+        f["FM"][:] = 0.0
+        f["FQ"][:] = 0.0
+        pr.forcing = f
+        s = oracle.OracleSession(pr, L)
+        nstep, nm1, n0, np1 = s.run(1)
+        t = s.state()["T"][:, n0]
+        s.close()
+        return t, pr.dt
+    t_ref, dt = run(-1)
+    t_f, _ = run(2)
+    r = (t_f - t_ref) / (dt * c)
+    return float(r.min()), float(r.max())
+
+
+def test_uniform_temperature_forcing_raises_t_by_dt_ft(oracle_lib):
+    lo, hi = uniform_ft_response(oracle_lib)
+    assert 0.99 < lo and hi < 1.01
+
+
+def test_uniform_forcing_detects_planted_sign(oracle_lib):
+    lo, hi = uniform_ft_response(oracle.plant(9))
+    assert not (0.99 < lo and hi < 1.01)
