@@ -4,6 +4,8 @@ Workload: ne30 (5400 elements), np4, L72, 40 tracers, JW baroclinic jet with
 perturbation, dt = 300 s, hyperviscosity subcycled 3x (nu = 1e15, nu_top =
 2.5e5), remap every step, limiter 8, PPM remap alg 1, no forcing.  One step =
 one prim_run_subcycle_c call through the C-ABI with all state resident in HBM.
+The same steps are then run in the mixed-precision form from the same state
+and its speed and per-field error against the FP64 run are reported.
 
 Prints one JSON line: SYPD = dt / (365 * seconds per step), the roofline of the
 dominant kernel (HIP-event durations on the library stream over the timed
@@ -11,7 +13,7 @@ steps; algorithmic bytes from the kernel's array traffic), the whole-step
 fraction of HBM peak under the canonical bytes-per-step model, and a CPU
 baseline (the oracle, OpenMP over elements, one step of the same workload).
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-mixed]
 """
 import argparse
 import json
@@ -77,16 +79,19 @@ def progress(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(args):
+def make_problem(args):
+    from hommexx_amd.problem import build_problem
+    return build_problem(ne=args.ne, nlev=args.nlev, qsize=args.qsize, seed=1, dt=args.dt, nu=1e15, nu_p=1e15,
+                         nu_q=1e15, nu_s=1e15, nu_div=1e15, nu_top=2.5e5, hypervis_subcycle=3, rsplit=1)
+
+
+def run_gpu(args, prob, form="fp64"):
     import numpy as np
     from hommexx_amd.hip import HipSession
-    from hommexx_amd.problem import build_problem
 
-    prob = build_problem(ne=args.ne, nlev=args.nlev, qsize=args.qsize, seed=1, dt=args.dt, nu=1e15, nu_p=1e15,
-                         nu_q=1e15, nu_s=1e15, nu_div=1e15, nu_top=2.5e5, hypervis_subcycle=3, rsplit=1)
-    progress("problem built: nelem %d nlev %d qsize %d" % (prob.nelem, prob.nlev, prob.qsize))
+    progress("problem: nelem %d nlev %d qsize %d, form %s" % (prob.nelem, prob.nlev, prob.qsize, form))
     sph = prob.metric["spheremp"]
-    with HipSession(prob, abort_on_error=True) as h:
+    with HipSession(prob, abort_on_error=True, form=form) as h:
         progress("state resident on the GPU")
         h.run(args.warmup)
         h.synchronize()
@@ -108,7 +113,7 @@ def run_gpu(args):
     dry = float((sph * st["ps_v"][:, n0]).sum())
     q0 = np.einsum("epq,eclpq->c", sph, prob.state["Qdp"][:, 0, :args.qsize])
     q1 = np.einsum("epq,eclpq->c", sph, st["Qdp"][:, nstep % 2, :args.qsize])
-    return prob, sec, prof, abs(dry / dry0 - 1.0), float(np.max(np.abs(q1 / q0 - 1.0)))
+    return sec, prof, abs(dry / dry0 - 1.0), float(np.max(np.abs(q1 / q0 - 1.0))), st
 
 
 def run_cpu(prob, min_seconds=10.0, max_steps=20):
@@ -152,6 +157,7 @@ def main():
     ap.add_argument("--qsize", type=int, default=40)
     ap.add_argument("--dt", type=float, default=300.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -162,8 +168,19 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
-    prob, sec, prof, dry_drift, q_drift = run_gpu(args)
+    prob = make_problem(args)
+    sec, prof, dry_drift, q_drift, st64 = run_gpu(args, prob, "fp64")
     nelem, nlev, q = prob.nelem, prob.nlev, args.qsize
+    mixed = None
+    if not args.no_mixed:
+        import numpy as np
+        msec, _, mdry, mq, stm = run_gpu(args, prob, "mixed")
+        err = {k: float(np.linalg.norm((stm[k] - st64[k]).ravel()) / np.linalg.norm(st64[k].ravel()))
+               for k in ("v", "T", "dp3d", "ps_v", "Qdp", "Q")}
+        mixed = {"ms_per_step": round(msec * 1e3, 3), "value": round(args.dt / (365.0 * msec), 3),
+                 "speedup_vs_fp64": round(sec / msec, 4), "max_rel_l2_vs_fp64": max(err.values()),
+                 "rel_l2_vs_fp64": err, "steps_compared": args.warmup + args.steps,
+                 "dry_mass_rel_drift": mdry, "tracer_mass_rel_drift_max": mq}
     sypd = args.dt / (365.0 * sec)
     # dominant kernel by total time
     kern = {k: v for k, v in prof.items() if kernel_bytes(k, nelem, nlev, q, 1) is not None}
@@ -196,6 +213,7 @@ def main():
                    "nelem": nelem, "nlev": nlev, "qsize": q, "form": "fp64"},
         "roofline": roof, "step_roofline": step_roof, "cpu_baseline": cpu,
         "conservation": {"dry_mass_rel_drift": dry_drift, "tracer_mass_rel_drift_max": q_drift},
+        "mixed": mixed,
         "phases": phases,
     }
     print(json.dumps(out))

@@ -85,6 +85,9 @@ struct Ctx {
   int *err_h = nullptr;
   // time levels (0-based)
   int nm1 = 0, n0 = 1, np1 = 2, nstep = 0, nstep0 = 0, n0_qdp = 0, np1_qdp = 1;
+  // 0: FP64 step; 1: mixed form (FP32 sphere operators in hyperviscosity and
+  // the tracer flux divergence / biharmonic)
+  int form = 0;
   // debugging: stop prim_run_subcycle_c after a phase (0: never)
   int debug_stop = 0;
   bool stopped = false;
@@ -393,6 +396,7 @@ void hyperviscosity(int np1, double dt, double eta_ave_w) {
   a.nu_ratio1 = c.p.nu_ratio1;
   a.nu_ratio2 = c.p.nu_ratio2;
   a.nsub = c.p.hypervis_subcycle;
+  a.mixed = c.form == 1;
   for (int cyc = 0; cyc < c.p.hypervis_subcycle; ++cyc) {
     Scope s("hv_cycle");
     launch_hv_cycle(c.st, a);
@@ -441,6 +445,7 @@ void advec_tracers(double dt) {
   a.nu_p = c.p.nu_p;
   a.nu_q = c.p.nu_q;
   a.limiter_option = c.p.limiter_option;
+  a.mixed = c.form == 1;
   a.rhs_viss = 0.0;
   const double dts = dt / 2.0;
   a.dt = dts;
@@ -1013,6 +1018,12 @@ int hommexx_hip_configure(int nlev, int qsize_d) {
 }
 
 void hommexx_hip_set_abort_on_error(int on) { C().abort_on_error = on != 0; }
+
+int hommexx_hip_set_form(int form) {
+  if (form != 0 && form != 1) return 1;
+  C().form = form;
+  return 0;
+}
 // Debugging aid: make prim_run_subcycle_c return after phase n (1: first RK
 // stage, 2: all RK stages, 3: hyperviscosity, 4: first tracer stage,
 // 5: tracers); 0 disables.  Time levels are not rotated when it stops.

@@ -20,6 +20,30 @@ __global__ __launch_bounds__(256) void dss_kernel(DssArgs a) {
   const int s0 = src[0], s1 = src[1], s2 = src[2];
   const double r = a.rsmp ? a.rsmp[(size_t)ie * 16 + pt] : 1.0;
   double x[DSS_MAX_FIELDS];
+  if (a.f32) {
+    // FP32 fields of the mixed form, accumulated in FP64
+#pragma unroll
+    for (int f = 0; f < DSS_MAX_FIELDS; ++f) {
+      if (f < a.nf) {
+        const DssField &F = a.f[f];
+        const float *in = (const float *)F.in;
+        const size_t off = (size_t)k * F.lstride;
+        double y = in[(size_t)ie * F.estride + off + pt];
+        if (s0 >= 0) y += (double)in[(size_t)(s0 >> 4) * F.estride + off + (s0 & 15)];
+        if (s1 >= 0) y += (double)in[(size_t)(s1 >> 4) * F.estride + off + (s1 & 15)];
+        if (s2 >= 0) y += (double)in[(size_t)(s2 >> 4) * F.estride + off + (s2 & 15)];
+        x[f] = y;
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < DSS_MAX_FIELDS; ++f) {
+      if (f < a.nf) {
+        const DssField &F = a.f[f];
+        ((float *)F.out)[(size_t)ie * F.estride + (size_t)k * F.lstride + pt] = (float)(a.rsmp ? x[f] * r : x[f]);
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int f = 0; f < DSS_MAX_FIELDS; ++f) {
     if (f < a.nf) {

@@ -151,6 +151,7 @@ __device__ __forceinline__ void limit9_slab(double minp, double maxp, double &x,
 // Setup phase (:585-626) + tracer phase (:571-687) for every tracer of an
 // (element, level) slab.  At stage 2 the qmin/qmax update of that stage
 // (min/max with the previous bounds, no neighbour exchange) runs first.
+template <bool MIXED>
 __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stage) {
   size_t slab;
   int ie, k, pt;
@@ -173,6 +174,12 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
   a.fdss_pre[((size_t)ie * 16 + pt) * L + k] = a.fdss[s] * e.sph;  // [ie][pt][lev]
   const bool add_hv = a.rhs_viss != 0.0;
   const bool limit = a.limiter_option == 8 || a.limiter_option == 9;
+  DvvLaneF df;
+  float rmd = 0.f;
+  if (MIXED) {
+    load_dvvf(a.m.dvv, pt, df);
+    rmd = (float)(1.0 / e.metdet * RREARTH);
+  }
   const double c = e.sph * dpdissk;
   const double s1 = limit ? slab_sum_serial(c) : 0.0;  // same for every tracer
   for (int q = 0; q < Q; ++q) {
@@ -185,8 +192,20 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
       minp = dmin(minp, slab_min(xq));
       maxp = dmax(maxp, slab_max(xq));
     }
-    const double q0 = add_hv ? a.qbih[qi] : 0;
-    double ptens = sph_div_update(e, d, -a.dt, vs0, vs1, qd, q0);
+    double ptens;
+    if (MIXED) {
+      // flux divergence in FP32 with compensated contractions; the update
+      // qdp - dt*div + qbih stays FP64
+      const double q0 = add_hv ? (double)((const float *)a.qbih)[qi] : 0;
+      const double fa = vs0 * qd, fb = vs1 * qd;
+      const float gv0 = (float)((e.Dinv[0] * fa + e.Dinv[2] * fb) * e.metdet);
+      const float gv1 = (float)((e.Dinv[1] * fa + e.Dinv[3] * fb) * e.metdet);
+      const float dv = (row_contractf(df.J, gv0) + col_contractf(df.I, gv1)) * rmd;
+      ptens = (qd + -a.dt * (double)dv + q0);
+    } else {
+      const double q0 = add_hv ? a.qbih[qi] : 0;
+      ptens = sph_div_update(e, d, -a.dt, vs0, vs1, qd, q0);
+    }
     bool wmin = stage == 2, wmax = stage == 2;
     if (limit) {
       double x = ptens / dpdissk;
@@ -254,12 +273,14 @@ __device__ __forceinline__ bool xchg_lane(int nelem, int nlev, XLane &x) {
 
 // gather of point (ie, pt) at level k from a levels-fastest field with
 // element stride es (own value, then the sources in the reference's order)
-__device__ __forceinline__ double gather_lf(const double *f, size_t es, size_t off, int L, int ie, int pt, int k,
-                                            int s0, int s1, int s2) {
+// (FP32 buffers of the mixed form are accumulated in FP64 the same way)
+template <typename T>
+__device__ __forceinline__ double gather_lf(const T *f, size_t es, size_t off, int L, int ie, int pt, int k, int s0,
+                                            int s1, int s2) {
   double x = f[(size_t)ie * es + off + (size_t)pt * L + k];
-  if (s0 >= 0) x += f[(size_t)(s0 >> 4) * es + off + (size_t)(s0 & 15) * L + k];
-  if (s1 >= 0) x += f[(size_t)(s1 >> 4) * es + off + (size_t)(s1 & 15) * L + k];
-  if (s2 >= 0) x += f[(size_t)(s2 >> 4) * es + off + (size_t)(s2 & 15) * L + k];
+  if (s0 >= 0) x += (double)f[(size_t)(s0 >> 4) * es + off + (size_t)(s0 & 15) * L + k];
+  if (s1 >= 0) x += (double)f[(size_t)(s1 >> 4) * es + off + (size_t)(s1 & 15) * L + k];
+  if (s2 >= 0) x += (double)f[(size_t)(s2 >> 4) * es + off + (size_t)(s2 & 15) * L + k];
   return x;
 }
 
@@ -305,6 +326,7 @@ __global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int avg) {
 // Stage 3 compute_dp + compute_qmin_qmax (rhs_multiplier 2, :406-485) and the
 // first half of the tracer biharmonic (:196-267): laplace(Q * dpdiss_ave / dp0)
 // for every tracer of an (element, level) slab.
+template <bool MIXED>
 __global__ __launch_bounds__(256) void euler_qmm_lap_kernel(EulerArgs a) {
   size_t slab;
   int ie, k, pt;
@@ -319,6 +341,12 @@ __global__ __launch_bounds__(256) void euler_qmm_lap_kernel(EulerArgs a) {
   const double dp = a.derived_dp[s] - rhsmdt * a.divdp_proj[s];
   const double wdp = a.dpdiss_ave[s];
   const double dp0 = a.dp0[k];
+  ElemLaneF ef;
+  DvvLaneF df;
+  if (MIXED) {
+    load_elemf(a.m, ie, pt, ef);
+    load_dvvf(a.m.dvv, pt, df);
+  }
   for (int q = 0; q < Q; ++q) {
     const size_t qi = (((size_t)ie * Q + q) * L + k) * 16 + pt;
     const double x = a.qdp_n0[qi] / dp;
@@ -330,7 +358,9 @@ __global__ __launch_bounds__(256) void euler_qmm_lap_kernel(EulerArgs a) {
     }
     double y = x;
     if (a.nu_p > 0) y = y * wdp / dp0;
-    a.qbih_pre[(((size_t)ie * Q + q) * 16 + pt) * L + k] = sph_laplace(e, d, y);  // [ie][q][pt][lev]
+    const size_t o = (((size_t)ie * Q + q) * 16 + pt) * L + k;  // [ie][q][pt][lev]
+    if (MIXED) ((float *)a.qbih_pre)[o] = sph_laplacef(ef, df, (float)y);
+    else a.qbih_pre[o] = sph_laplace(e, d, y);
   }
 }
 
@@ -338,6 +368,7 @@ __global__ __launch_bounds__(256) void euler_qmm_lap_kernel(EulerArgs a) {
 // rspheremp), second laplacian, times -rhs_viss*dt*nu_q*dp0/spheremp, for
 // tracers [QB*blockIdx.y, QB*blockIdx.y + QB).  After the LDS transpose the
 // 16-lane groups are slabs (one level, 16 points) for the laplacian.
+template <bool MIXED>
 __global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
   __shared__ double tile[2][XL][17];
   XLane x;
@@ -353,11 +384,20 @@ __global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
   double v[QB];
 #pragma unroll
   for (int i = 0; i < QB; ++i)
-    if (i < nq && gk) v[i] = gather_lf(a.qbih_pre, es, (size_t)(q0 + i) * 16 * L, L, x.ie, x.pt, x.k, s0, s1, s2);
+    if (i < nq && gk) {
+      if (MIXED) v[i] = gather_lf((const float *)a.qbih_pre, es, (size_t)(q0 + i) * 16 * L, L, x.ie, x.pt, x.k, s0, s1, s2);
+      else v[i] = gather_lf(a.qbih_pre, es, (size_t)(q0 + i) * 16 * L, L, x.ie, x.pt, x.k, s0, s1, s2);
+    }
   ElemLane e;
   load_elem_light(a.m, x.ie, x.wpt, e);
   DvvLane d;
   load_dvv(a.m.dvv, x.wpt, d);
+  ElemLaneF ef;
+  DvvLaneF df;
+  if (MIXED) {
+    load_elemf(a.m, x.ie, x.wpt, ef);
+    load_dvvf(a.m.dvv, x.wpt, df);
+  }
   const double f = -a.rhs_viss * a.dt * a.nu_q;
   const double dp0 = wkv ? a.dp0[wk] : 0.0;
 #pragma unroll
@@ -368,8 +408,14 @@ __global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
     __syncthreads();
     // every lane of a slab takes part in the laplacian's cross-lane reads
     const double y = wkv ? t[x.wk][x.wpt] * e.rsmp : 0.0;
-    const double lap = sph_laplace(e, d, y);
-    if (wkv) a.qbih[(((size_t)x.ie * Q + q0 + i) * L + wk) * 16 + x.wpt] = (f * dp0 * lap / e.sph);
+    const size_t o = (((size_t)x.ie * Q + q0 + i) * L + wk) * 16 + x.wpt;
+    if (MIXED) {
+      const double lap = sph_laplacef(ef, df, (float)y);
+      if (wkv) ((float *)a.qbih)[o] = (float)(f * dp0 * lap / e.sph);
+    } else {
+      const double lap = sph_laplace(e, d, y);
+      if (wkv) a.qbih[o] = (f * dp0 * lap / e.sph);
+    }
   }
 }
 
@@ -385,7 +431,8 @@ void launch_euler_nbr_minmax(hipStream_t st, const EulerArgs &a) {
 }
 
 void launch_euler_advect(hipStream_t st, const EulerArgs &a, int stage) {
-  hipLaunchKernelGGL(euler_advect_kernel, dim3(slab_blocks(a)), dim3(256), 0, st, a, stage);
+  if (a.mixed) hipLaunchKernelGGL(euler_advect_kernel<true>, dim3(slab_blocks(a)), dim3(256), 0, st, a, stage);
+  else hipLaunchKernelGGL(euler_advect_kernel<false>, dim3(slab_blocks(a)), dim3(256), 0, st, a, stage);
 }
 
 void launch_euler_dss(hipStream_t st, const EulerArgs &a, int avg) {
@@ -394,13 +441,15 @@ void launch_euler_dss(hipStream_t st, const EulerArgs &a, int avg) {
 }
 
 void launch_euler_qmm_lap(hipStream_t st, const EulerArgs &a) {
-  hipLaunchKernelGGL(euler_qmm_lap_kernel, dim3(slab_blocks(a)), dim3(256), 0, st, a);
+  if (a.mixed) hipLaunchKernelGGL(euler_qmm_lap_kernel<true>, dim3(slab_blocks(a)), dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(euler_qmm_lap_kernel<false>, dim3(slab_blocks(a)), dim3(256), 0, st, a);
 }
 
 void launch_euler_bih_post(hipStream_t st, const EulerArgs &a) {
   if (a.qsize <= 0) return;
-  hipLaunchKernelGGL(euler_bih_post_kernel, dim3(xchg_blocks_x(a.nelem, a.nlev), (a.qsize + QB - 1) / QB), dim3(256),
-                     0, st, a);
+  const dim3 g(xchg_blocks_x(a.nelem, a.nlev), (a.qsize + QB - 1) / QB);
+  if (a.mixed) hipLaunchKernelGGL(euler_bih_post_kernel<true>, g, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(euler_bih_post_kernel<false>, g, dim3(256), 0, st, a);
 }
 
 }  // namespace hx

@@ -7,6 +7,8 @@
 //               nu_top sponge on the top three levels, dptens <- dp3d*sph + dt*dptens
 //   DSS
 //   update    : apply the inverse mass, v += , T += - heating/cp, dp3d = dptens*rsmp
+// The mixed form swaps in FP32 laplacian kernels (below) and an FP32 first
+// exchange; the second exchange and the update are the same.
 #include "kernels.hpp"
 
 namespace hx {
@@ -82,6 +84,79 @@ __global__ __launch_bounds__(256) void hv_lap2_pre_kernel(HvArgs a) {
   a.dpt_a[s] = dpt;
 }
 
+// ---------------------------------------------------------- mixed form
+// The two laplacians in FP32 (compensated contractions).  The first
+// laplacian's outputs and their assembled values are FP32 arrays held in the
+// first halves of vt_a/tt_a/dpt_a and vt_b/tt_b/dpt_b; T and dp3d are shifted
+// by a per-slab reference value before rounding (laplacians annihilate
+// constants).  Accumulators, scaling, the sponge combination and everything
+// after stay FP64.
+__global__ __launch_bounds__(256) void hv_lap1_mixed_kernel(HvArgs a) {
+  int ie, k, pt;
+  if (!slab_of(gridDim.x, a.nelem, a.nlev, ie, k, pt)) return;
+  ElemLaneF e;
+  load_elemf(a.m, ie, pt, e);
+  DvvLaneF d;
+  load_dvvf(a.m.dvv, pt, d);
+  const size_t s = ((size_t)ie * a.nlev + k) * 16 + pt;
+  const size_t v = ((size_t)ie * a.nlev + k) * 32 + pt;
+  const double t = a.t[s], dp = a.dp[s];
+  ((float *)a.tt_a)[s] = sph_laplacef(e, d, (float)(t - slab_ref(t)));
+  ((float *)a.dpt_a)[s] = sph_laplacef(e, d, (float)(dp - slab_ref(dp)));
+  float l0, l1;
+  sph_vlaplace_contraf(e, d, (float)a.nu_ratio1, (float)a.v[v], (float)a.v[v + 16], l0, l1);
+  ((float *)a.vt_a)[v] = l0;
+  ((float *)a.vt_a)[v + 16] = l1;
+}
+
+__global__ __launch_bounds__(256) void hv_lap2_pre_mixed_kernel(HvArgs a) {
+  int ie, k, pt;
+  if (!slab_of(gridDim.x, a.nelem, a.nlev, ie, k, pt)) return;
+  ElemLaneF e;
+  load_elemf(a.m, ie, pt, e);
+  DvvLaneF d;
+  load_dvvf(a.m.dvv, pt, d);
+  const double sph = a.m.spheremp[(size_t)ie * 16 + pt];
+  const size_t s = ((size_t)ie * a.nlev + k) * 16 + pt;
+  const size_t v = ((size_t)ie * a.nlev + k) * 32 + pt;
+  double tt = sph_laplacef(e, d, ((const float *)a.tt_b)[s]);
+  double dpt = sph_laplacef(e, d, ((const float *)a.dpt_b)[s]);
+  float f0, f1;
+  sph_vlaplace_contraf(e, d, (float)a.nu_ratio2, ((const float *)a.vt_b)[v], ((const float *)a.vt_b)[v + 16], f0, f1);
+  double vt0 = f0, vt1 = f1;
+  const double dpn = a.dp[s];
+  a.dpdiss_ave[s] += a.eta_ave_w * dpn / a.nsub;
+  a.dpdiss_bih[s] += a.eta_ave_w * dpt / a.nsub;
+  double lv0 = 0, lv1 = 0, lt = 0, ldp = 0;
+  const bool top = a.nu_top > 0 && k < 3;
+  if (top) {
+    float g0, g1;
+    sph_vlaplace_contraf(e, d, 1.f, (float)a.v[v], (float)a.v[v + 16], g0, g1);
+    lv0 = g0;
+    lv1 = g1;
+    const double t = a.t[s];
+    lt = sph_laplacef(e, d, (float)(t - slab_ref(t)));
+    ldp = sph_laplacef(e, d, (float)(dpn - slab_ref(dpn)));
+  }
+  vt0 *= -a.nu;
+  vt1 *= -a.nu;
+  tt *= -a.nu_s;
+  dpt *= -a.nu_p;
+  if (top) {
+    const double sc = (k == 0 ? 4.0 : k == 1 ? 2.0 : 1.0) * a.nu_top;
+    vt0 += sc * lv0;
+    vt1 += sc * lv1;
+    tt += sc * lt;
+    dpt += sc * ldp;
+  }
+  dpt *= a.dt;
+  dpt += dpn * sph;
+  a.vt_a[v] = vt0;
+  a.vt_a[v + 16] = vt1;
+  a.tt_a[s] = tt;
+  a.dpt_a[s] = dpt;
+}
+
 __global__ __launch_bounds__(256) void hv_update_kernel(HvArgs a) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (size_t)a.nelem * a.nlev * 16) return;
@@ -104,7 +179,8 @@ __global__ __launch_bounds__(256) void hv_update_kernel(HvArgs a) {
 void launch_hv_cycle(hipStream_t st, const HvArgs &a) {
   const size_t nslab = (size_t)a.nelem * a.nlev;
   const int blocks = (int)((nslab + 15) / 16);
-  hipLaunchKernelGGL(hv_lap1_kernel, dim3(blocks), dim3(256), 0, st, a);
+  if (a.mixed) hipLaunchKernelGGL(hv_lap1_mixed_kernel, dim3(blocks), dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(hv_lap1_kernel, dim3(blocks), dim3(256), 0, st, a);
   DssArgs x{};
   x.nf = 4;
   x.nelem = a.nelem;
@@ -116,9 +192,18 @@ void launch_hv_cycle(hipStream_t st, const HvArgs &a) {
   x.f[1] = {a.vt_a + 16, a.vt_b + 16, ev, 32};
   x.f[2] = {a.tt_a, a.tt_b, es, 16};
   x.f[3] = {a.dpt_a, a.dpt_b, es, 16};
+  x.f32 = a.mixed;
+  if (a.mixed) {
+    // the first exchange's buffers hold floats: the second velocity
+    // component starts 16 floats in
+    x.f[1] = {(const double *)((const float *)a.vt_a + 16), (double *)((float *)a.vt_b + 16), ev, 32};
+  }
   launch_dss(st, x);
-  hipLaunchKernelGGL(hv_lap2_pre_kernel, dim3(blocks), dim3(256), 0, st, a);
+  if (a.mixed) hipLaunchKernelGGL(hv_lap2_pre_mixed_kernel, dim3(blocks), dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(hv_lap2_pre_kernel, dim3(blocks), dim3(256), 0, st, a);
   x.rsmp = nullptr;
+  x.f32 = 0;
+  x.f[1] = {a.vt_a + 16, a.vt_b + 16, ev, 32};
   launch_dss(st, x);
   const size_t n = nslab * 16;
   hipLaunchKernelGGL(hv_update_kernel, dim3((n + 255) / 256), dim3(256), 0, st, a);

@@ -7,7 +7,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "sphere.hpp"
+#include "sphere32.hpp"
 
 namespace hx {
 
@@ -32,6 +32,7 @@ struct DssField {
 struct DssArgs {
   DssField f[DSS_MAX_FIELDS];
   int nf, nelem, nlev;
+  int f32;               // fields hold FP32 values (accumulated in FP64)
   const int *src;        // [ie][16][3] global point ids (ie*16 + pt) or -1
   const double *rsmp;    // multiply by rspheremp if non-null
 };
@@ -46,6 +47,7 @@ struct HvArgs {
   double *dpdiss_ave, *dpdiss_bih;
   double dt, eta_ave_w, nu, nu_s, nu_p, nu_top, nu_ratio1, nu_ratio2;
   int nsub;
+  int mixed;                        // FP32 laplacians, FP32 first-laplacian buffers
 };
 
 struct EulerArgs {
@@ -63,6 +65,7 @@ struct EulerArgs {
   double dt, rhs_multiplier, rhs_viss, nu_p, nu_q;
   int limiter_option;
   const double *qdp_avg_n0;   // stage 3: Qdp(n0_qdp) for the time average, else nullptr
+  int mixed;                  // FP32 flux divergence and biharmonic (qbih, qbih_pre hold floats)
 };
 
 struct RemapArgs {

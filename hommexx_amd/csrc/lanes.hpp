@@ -157,3 +157,65 @@ __device__ __forceinline__ double col_contract(const double c[4], double x) {
 }
 
 }  // namespace hx
+
+// ------------------------------------------------------------------ FP32
+// Single-precision counterparts for the mixed form: one DPP / swizzle per
+// value, and dot products with compensated accumulation (exact product
+// error from an FMA, exact sum error from TwoSum, summed separately and
+// added at the end: the Dot2 scheme of Ogita, Rump and Oishi), which keeps a
+// 4-term contraction accurate to a few FP32 ulps of its result.
+namespace hx {
+
+template <int K>
+__device__ __forceinline__ float row_getf(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), K * 0x55, 0xF, 0xF, false));
+}
+template <int K>
+__device__ __forceinline__ float col_getf(float x) {
+  constexpr int pattern = 0x13 | ((K << 2) << 5);
+  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(x), pattern));
+}
+
+// s + e accumulates c*x exactly up to the final rounding
+__device__ __forceinline__ void dot2_step(float c, float x, float &s, float &e) {
+  const float p = c * x;
+  const float pe = __builtin_fmaf(c, x, -p);  // c*x - p exactly
+  const float t = s + p;
+  const float z = t - s;
+  const float te = (s - (t - z)) + (p - z);  // s + p - t exactly (TwoSum)
+  s = t;
+  e += pe + te;
+}
+
+__device__ __forceinline__ float row_contractf(const float c[4], float x) {
+  float s = 0.f, e = 0.f;
+  dot2_step(c[0], row_getf<0>(x), s, e);
+  dot2_step(c[1], row_getf<1>(x), s, e);
+  dot2_step(c[2], row_getf<2>(x), s, e);
+  dot2_step(c[3], row_getf<3>(x), s, e);
+  return s + e;
+}
+__device__ __forceinline__ float col_contractf(const float c[4], float x) {
+  float s = 0.f, e = 0.f;
+  dot2_step(c[0], col_getf<0>(x), s, e);
+  dot2_step(c[1], col_getf<1>(x), s, e);
+  dot2_step(c[2], col_getf<2>(x), s, e);
+  dot2_step(c[3], col_getf<3>(x), s, e);
+  return s + e;
+}
+
+struct DvvLaneF {
+  float J[4], I[4], TJ[4], TI[4];
+};
+__device__ __forceinline__ void load_dvvf(const double *__restrict__ dvv, int pt, DvvLaneF &d) {
+  const int ig = pt >> 2, jg = pt & 3;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    d.J[k] = (float)dvv[jg * 4 + k];
+    d.I[k] = (float)dvv[ig * 4 + k];
+    d.TJ[k] = (float)dvv[k * 4 + jg];
+    d.TI[k] = (float)dvv[k * 4 + ig];
+  }
+}
+
+}  // namespace hx

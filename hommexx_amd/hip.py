@@ -45,6 +45,7 @@ def load():
         L.hommexx_hip_set_abort_on_error.argtypes = [_I]
         L.hommexx_hip_profile.argtypes = [_I]
         L.hommexx_hip_debug_stop.argtypes = [_I]
+        L.hommexx_hip_set_form.argtypes = [_I]
         _lib = L
     return _lib
 
@@ -69,13 +70,17 @@ class HipSession:
 
     _alive = False
 
-    def __init__(self, prob, abort_on_error=False):
+    FORMS = {"fp64": 0, "mixed": 1}
+
+    def __init__(self, prob, abort_on_error=False, form="fp64"):
         L = load()
         if HipSession._alive:
             raise HommexxError("a HipSession is already active; close() it first")
         self.prob = prob
         self.L = L
         L.hommexx_hip_set_abort_on_error(1 if abort_on_error else 0)
+        if L.hommexx_hip_set_form(self.FORMS[form]):
+            raise HommexxError("unknown form %r" % form)
         L.hommexx_hip_clear_error()
         L.initialize_hommexx_session()
         self._check("initialize_hommexx_session")
@@ -187,6 +192,7 @@ class HipSession:
 
     def close(self):
         if not getattr(self, "_closed", True):
+            self.L.hommexx_hip_set_form(0)
             self.L.finalize_hommexx_session()
             self._closed = True
             HipSession._alive = False

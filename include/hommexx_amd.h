@@ -110,6 +110,12 @@ void f90_push_forcing_to_cxx(double *FM, double *FT, double *FQ, double *Qdp);
 int hommexx_hip_configure(int nlev, int qsize_d);
 /* 1 (default): abort like the reference; 0: record the error and return */
 void hommexx_hip_set_abort_on_error(int on);
+/* 0 (default): the FP64 step.  1: the mixed-precision form -- the sphere
+ * operators of hyperviscosity, of the tracer flux divergence and of the
+ * tracer biharmonic run in FP32 with compensated contractions; column scans,
+ * remap, DSS accumulation, limiters and RK combinations stay FP64.  Takes
+ * effect at the next prim_run_subcycle_c.  Returns 1 for an unknown form. */
+int hommexx_hip_set_form(int form);
 /* debugging aid: prim_run_subcycle_c returns after phase n (1: first RK stage,
  * 2: RK stages, 3: hyperviscosity, 4: first tracer stage, 5: tracers), 0: off */
 void hommexx_hip_debug_stop(int phase);

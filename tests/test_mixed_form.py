@@ -1,0 +1,69 @@
+"""The mixed-precision form (hommexx_hip_set_form(1)) against the FP64 step
+of the same library on identical inputs.
+
+Bar: every prognostic field within 1e-6 relative L2 of the FP64 run (the
+mixed form's accuracy requirement); the dry mass stays conserved to FP64
+rounding (dp3d's biharmonic is assembled and applied in FP64) and the tracer
+mass drift is bounded by the FP32 flux-divergence rounding.
+"""
+import numpy as np
+import pytest
+
+from hommexx_amd.hip import HipSession
+from hommexx_amd.problem import build_problem
+from parity import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+PROGNOSTIC = ("v", "T", "dp3d", "ps_v", "Qdp", "Q")
+
+
+def run_form(prob, form, nsteps):
+    with HipSession(prob, form=form) as h:
+        tl = h.run(nsteps)
+        return h.state(), tl
+
+
+def errors(a, b):
+    return {k: rel_l2(a[k], b[k]) for k in PROGNOSTIC}
+
+
+@pytest.mark.parametrize("ne,nlev,qsize,nsteps", [(4, 26, 4, 10), (8, 72, 4, 10)])
+def test_mixed_form_within_1e6_of_fp64(ne, nlev, qsize, nsteps):
+    prob = build_problem(ne=ne, nlev=nlev, qsize=qsize, seed=21, t_noise=0.1)
+    ref, tl = run_form(prob, "fp64", nsteps)
+    mix, tl2 = run_form(prob, "mixed", nsteps)
+    assert tl == tl2
+    err = errors(mix, ref)
+    print(ne, nlev, {k: "%.2e" % v for k, v in err.items()})
+    for k, e in err.items():
+        assert e <= 1e-6, (k, e)
+    # the forms differ: the FP32 operators are really in use
+    assert max(err.values()) > 1e-14
+
+
+def test_mixed_form_mass():
+    prob = build_problem(ne=8, nlev=26, qsize=2, seed=22)
+    sph = prob.metric["spheremp"]
+    with HipSession(prob, form="mixed") as h:
+        st0 = h.state()
+        nstep, nm1, n0, np1 = h.run(5)
+        st = h.state()
+    dry0 = (sph * st0["ps_v"][:, 1]).sum()
+    dry = (sph * st["ps_v"][:, n0]).sum()
+    q0 = np.einsum("epq,eclpq->c", sph, st0["Qdp"][:, 0])[:2]
+    q = np.einsum("epq,eclpq->c", sph, st["Qdp"][:, nstep % 2])[:2]
+    print("dry drift %.2e, tracer drift %s" % (abs(dry / dry0 - 1), np.abs(q / q0 - 1)))
+    assert abs(dry / dry0 - 1) < 1e-13
+    assert np.all(np.abs(q / q0 - 1) < 1e-8)
+
+
+def test_form_switch_returns_to_fp64():
+    """After a mixed session, a new session is FP64 again and bit-identical
+    to a fresh FP64 run."""
+    prob = build_problem(ne=4, nlev=26, qsize=2, seed=23)
+    a, _ = run_form(prob, "fp64", 2)
+    run_form(prob, "mixed", 1)
+    b, _ = run_form(prob, "fp64", 2)
+    for k in PROGNOSTIC:
+        assert np.array_equal(a[k], b[k]), k
