@@ -49,6 +49,8 @@ def _bind(path):
     L.oracle_unit_vorticity.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
     L.oracle_unit_laplace.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
     L.oracle_unit_vlaplace.argtypes = [ctypes.c_int, ctypes.c_double] + [_dp] * 9
+    L.oracle_unit_laplace_tensor.argtypes = [ctypes.c_int] + [_dp] * 6
+    L.oracle_unit_vlaplace_cartesian.argtypes = [ctypes.c_int] + [_dp] * 7
     L.oracle_unit_limiter.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
     L.oracle_unit_ppm.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]
     L.oracle_unit_dss.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int]

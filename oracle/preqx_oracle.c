@@ -29,7 +29,11 @@
  *   DSS               continuity of every node, DSS(spheremp c) rspheremp == c,
  *                     54 ne^2 + 2 unique nodes
  *   CAAR              the steady JW jet stays balanced, converging with ne
- *   hyperviscosity    damping of a Y2 harmonic at dt nu (6/a^2)^2 per subcycle
+ *   hyperviscosity    damping of a Y2 harmonic at dt nu (6/a^2)^2 per subcycle;
+ *                     tensor form: identity tensor == simple laplacian bit for
+ *                     bit, cartesian vector laplacian leaves rigid rotation
+ *                     undamped and agrees with the contravariant form, both to
+ *                     a truncation error converging at second order
  *   Euler step        constant mixing ratio kept; zonal tracer kept by the
  *                     zonal jet (third-order convergence); limiter bounds,
  *                     mass and 1-norm properties (preqx_ut.cpp:1335-1532)
@@ -339,6 +343,37 @@ static void vlaplace_sphere_wk_contra(const Oracle *o, int ie, int nl, double nu
       }
 }
 
+/* vlaplace_sphere_wk_cartesian (SphereOperators.hpp:750-814, UNDAMPRRCART):
+ * tensor laplacian of the three cartesian components, projected back, plus
+ * 2 spheremp v / a^2.  Output may alias input.  scratch: 7 element blocks. */
+static void vlaplace_sphere_wk_cartesian(const Oracle *o, int ie, int nl, const double *u0, const double *u1,
+                                         double *l0, double *l1, double *scr) {
+  const double *vs = o->vec_sph2cart + (size_t)ie * 6 * NP2; /* [2][3][NP][NP] */
+  const double *sph = E2(o->spheremp, ie);
+#define VS(c, i, ig, jg) vs[(((c) * 3 + (i)) * NP + (ig)) * NP + (jg)]
+  double *lc[3] = {scr, scr + E3, scr + 2 * E3};
+  double *w = scr + 3 * E3;
+  for (int ig = 0; ig < NP; ++ig)
+    for (int jg = 0; jg < NP; ++jg)
+      for (int k = 0; k < nl; ++k) {
+        const double v0 = AT(u0, ig, jg, k), v1 = AT(u1, ig, jg, k);
+        for (int i = 0; i < 3; ++i) AT(lc[i], ig, jg, k) = VS(0, i, ig, jg) * v0 + VS(1, i, ig, jg) * v1;
+      }
+  for (int i = 0; i < 3; ++i) laplace_tensor(o, ie, nl, lc[i], lc[i], w, w + E3, w + 2 * E3, w + 3 * E3);
+  for (int ig = 0; ig < NP; ++ig)
+    for (int jg = 0; jg < NP; ++jg)
+      for (int k = 0; k < nl; ++k) {
+        const double v0 = AT(u0, ig, jg, k), v1 = AT(u1, ig, jg, k);
+        const double a = AT(lc[0], ig, jg, k), b = AT(lc[1], ig, jg, k), c = AT(lc[2], ig, jg, k);
+        const double und = PLANT(11) ? 0.0 : 2.0; /* plant 11: undamping term dropped */
+        AT(l0, ig, jg, k) = VS(0, 0, ig, jg) * a + VS(0, 1, ig, jg) * b + VS(0, 2, ig, jg) * c +
+                            und * M2(sph, ig, jg) * v0 * (RREARTH) * (RREARTH);
+        AT(l1, ig, jg, k) = VS(1, 0, ig, jg) * a + VS(1, 1, ig, jg) * b + VS(1, 2, ig, jg) * c +
+                            und * M2(sph, ig, jg) * v1 * (RREARTH) * (RREARTH);
+      }
+#undef VS
+}
+
 /* ==================================================================== DSS
  * Boundary exchange on one rank (BoundaryExchange.cpp:285-618): every
  * element packs its edge/corner values into the neighbour's receive slot,
@@ -639,9 +674,10 @@ static void hv_run(Oracle *o, int np1, double dt_in, double eta_ave_w, double *w
         laplace_simple(o, ie, L, dpt, dpt, w, w + E3);
         vlaplace_sphere_wk_contra(o, ie, L, o->nu_ratio2, vt0, vt1, vt0, vt1, w);
       } else {
-        /* tensor hyperviscosity: the cartesian vector laplacian
-           (SphereOperators.hpp:750-814) is not restated here */
-        o->error = 3;
+        /* TagSecondLaplaceTensorHV (HyperviscosityFunctorImpl.hpp:110-128) */
+        laplace_tensor(o, ie, L, tt, tt, w, w + E3, w + 2 * E3, w + 3 * E3);
+        laplace_tensor(o, ie, L, dpt, dpt, w, w + E3, w + 2 * E3, w + 3 * E3);
+        vlaplace_sphere_wk_cartesian(o, ie, L, vt0, vt1, vt0, vt1, w);
       }
     }
     /* pre-exchange */
@@ -1241,7 +1277,6 @@ int oracle_set_params(Oracle *o, int remap_alg, int limiter_option, int rsplit, 
       hypervis_order != 2 || rsplit < 1 || qsplit < 1 || !(nu > 0) || !(nu_p > 0) || !(nu_div > 0) ||
       qsize != o->qsize || (ftype != -1 && ftype != 0 && ftype != 2))
     return 1;
-  if (hypervis_scaling != 0.0) return 4; /* tensor hyperviscosity is not restated */
   o->remap_alg = remap_alg; o->limiter_option = limiter_option; o->rsplit = rsplit; o->qsplit = qsplit;
   o->time_step_type = time_step_type; o->state_frequency = state_frequency;
   o->nu = nu; o->nu_p = nu_p; o->nu_q = nu_q; o->nu_s = nu_s; o->nu_div = nu_div; o->nu_top = nu_top;
@@ -1539,6 +1574,28 @@ void oracle_unit_vlaplace(int nlev, double nu_ratio, const double *dvv, const do
   Oracle *o = unit_oracle(nlev, dvv, D, Dinv, metdet, spheremp, mp, metinv);
   double *b = alloc0(6 * NP2 * nlev);
   vlaplace_sphere_wk_contra(o, 0, nlev, nu_ratio, v, v + NP2 * nlev, lap, lap + NP2 * nlev, b);
+  free(b);
+  oracle_destroy(o);
+}
+
+void oracle_unit_laplace_tensor(int nlev, const double *dvv, const double *Dinv, const double *spheremp,
+                                const double *tensorvisc, const double *s, double *lap) {
+  Oracle *o = unit_oracle(nlev, dvv, NULL, Dinv, NULL, spheremp, NULL, NULL);
+  memcpy(o->tensorvisc, tensorvisc, sizeof(double) * 64);
+  double *b = alloc0(4 * NP2 * nlev);
+  laplace_tensor(o, 0, nlev, s, lap, b, b + NP2 * nlev, b + 2 * NP2 * nlev, b + 3 * NP2 * nlev);
+  free(b);
+  oracle_destroy(o);
+}
+
+void oracle_unit_vlaplace_cartesian(int nlev, const double *dvv, const double *Dinv, const double *spheremp,
+                                    const double *tensorvisc, const double *vec_sph2cart, const double *v,
+                                    double *lap) {
+  Oracle *o = unit_oracle(nlev, dvv, NULL, Dinv, NULL, spheremp, NULL, NULL);
+  memcpy(o->tensorvisc, tensorvisc, sizeof(double) * 64);
+  memcpy(o->vec_sph2cart, vec_sph2cart, sizeof(double) * 96);
+  double *b = alloc0(7 * NP2 * nlev);
+  vlaplace_sphere_wk_cartesian(o, 0, nlev, v, v + NP2 * nlev, lap, lap + NP2 * nlev, b);
   free(b);
   oracle_destroy(o);
 }

@@ -22,6 +22,12 @@ then fail.
   exactly in interior cells.  Planted: wrong dma index in the edge values.
 * DSS: every node's copies agree after assembly.  Planted: corner copies
   dropped.
+* Tensor hyperviscosity (SphereOperators.hpp:602-635, 750-814): with an
+  identity tensor laplace_tensor equals laplace_simple bit for bit; the
+  assembled cartesian vector laplacian leaves rigid rotation undamped and
+  agrees with the contravariant form on a divergent field, both to a
+  truncation error converging at second order.  Planted: undamping term
+  dropped.
 * CAM forcing (CamForcing.cpp): after a forced step the tracer mass (and,
   moist, the surface-pressure mass) equals the initial mass plus the clipped
   dt*FQ, since the step itself conserves mass; a uniform FT raises T by
@@ -282,3 +288,55 @@ def test_uniform_temperature_forcing_raises_t_by_dt_ft(oracle_lib):
 def test_uniform_forcing_detects_planted_sign(oracle_lib):
     lo, hi = uniform_ft_response(oracle.plant(9))
     assert not (0.99 < lo and hi < 1.01)
+
+
+# ---------------------------------------------------------------- tensor HV
+def _el(a, ie):
+    return np.ascontiguousarray(a[ie], dtype=np.float64)
+
+
+IDENT = np.ascontiguousarray(np.stack([np.stack([np.ones((4, 4)), np.zeros((4, 4))]),
+                                       np.stack([np.zeros((4, 4)), np.ones((4, 4))])]))
+
+
+def tensor_checks(L, ne):
+    pr = build_problem(ne=ne, nlev=26, qsize=1)
+    m, lat, lon = pr.metric, pr.mesh.lat, pr.mesh.lon
+    rot = np.zeros((pr.nelem, 2, 4, 4))
+    dif = np.zeros((pr.nelem, 2, 4, 4))
+    bitdiff = 0.0
+    for ie in range(pr.nelem):
+        args = (P(pr.dvv), P(_el(m["Dinv"], ie)), P(_el(m["spheremp"], ie)))
+        f = np.ascontiguousarray(np.sin(lat[ie]) * np.cos(lon[ie]))
+        a, b = np.zeros(16), np.zeros(16)
+        L.oracle_unit_laplace(1, *args, P(f), P(a))
+        L.oracle_unit_laplace_tensor(1, *args, P(IDENT), P(f), P(b))
+        bitdiff = max(bitdiff, float(np.abs(a - b).max()))
+        v = np.ascontiguousarray(np.concatenate([np.cos(lat[ie]).ravel(), np.zeros(16)]))
+        lap = np.zeros(32)
+        L.oracle_unit_vlaplace_cartesian(1, *args, P(IDENT), P(_el(m["vec_sph2cart"], ie)), P(v), P(lap))
+        rot[ie] = lap.reshape(2, 4, 4)
+        w = np.ascontiguousarray(np.concatenate([np.zeros(16), np.sin(2 * lat[ie]).ravel()]))
+        l1, l2 = np.zeros(32), np.zeros(32)
+        L.oracle_unit_vlaplace_cartesian(1, *args, P(IDENT), P(_el(m["vec_sph2cart"], ie)), P(w), P(l1))
+        L.oracle_unit_vlaplace(1, 1.0, P(pr.dvv), P(_el(m["D"], ie)), P(_el(m["Dinv"], ie)),
+                               P(_el(m["metdet"], ie)), P(_el(m["spheremp"], ie)), P(_el(m["mp"], ie)),
+                               P(_el(m["metinv"], ie)), P(w), P(l2))
+        dif[ie] = (l1 - l2).reshape(2, 4, 4)
+    rs = pr.mesh.rspheremp
+    asm = lambda x: np.stack([pr.mesh.dss_scalar_reference(x[:, c]) * rs for c in range(2)], 1)
+    return bitdiff, float(np.abs(asm(rot)).max() * A * A), float(np.abs(asm(dif)).max() * A * A)
+
+
+def tensor_ok(L):
+    b4, r4, d4 = tensor_checks(L, 4)
+    b8, r8, d8 = tensor_checks(L, 8)
+    return bool(b4 == 0 and b8 == 0 and r4 < 0.1 and r4 > 3 * r8 and d4 < 0.3 and d4 > 3 * d8)
+
+
+def test_tensor_hyperviscosity_operators(oracle_lib):
+    assert tensor_ok(oracle_lib)
+
+
+def test_tensor_checks_detect_planted_undamping_drop(oracle_lib):
+    assert not tensor_ok(oracle.plant(11))
