@@ -535,7 +535,51 @@ static void caar_element(Oracle *o, int ie, int nm1, int n0, int np1, int n0_qdp
       integration = integration + divv[i];
     }
   }
-  /* phase 3: omega accumulation (rsplit > 0 path) */
+  /* phase 3, rsplit == 0 (:136-143, :237-300, :497-597): eta_dot_dpdn from
+     the column sums of div(v dp), vertical advection of T and v, and the
+     eta_dot_dpdn accumulation; zero vertical flux otherwise */
+  double *eta = w + 14 * E3, *tva = w + 15 * E3, *vva0 = w + 16 * E3, *vva1 = w + 17 * E3;
+  for (size_t i = 0; i < E3; ++i) eta[i] = tva[i] = vva0[i] = vva1[i] = 0.0;
+  if (o->rsplit == 0) {
+    for (int pt = 0; pt < NP2; ++pt) {
+      double *e = eta + (size_t)pt * L;
+      const double *dv = divv + (size_t)pt * L;
+      double sdot = 0.0;
+      for (int k = 0; k < L - 1; ++k) {
+        sdot += dv[k];
+        e[k + 1] = sdot;
+      }
+      sdot += dv[L - 1];
+      for (int k = 1; k < L; ++k) e[k] = o->hybi[k] * sdot - e[k];
+      e[0] = 0.0;
+      const double *dp = dpn + (size_t)pt * L, *t = tn + (size_t)pt * L;
+      const double *u = v0n + (size_t)pt * L, *vv = v1n + (size_t)pt * L;
+      double *ta = tva + (size_t)pt * L, *ua = vva0 + (size_t)pt * L, *va = vva1 + (size_t)pt * L;
+      {
+        const double facp = (0.5 * 1 / dp[0]) * e[1];
+        ta[0] = facp * (t[1] - t[0]);
+        ua[0] = facp * (u[1] - u[0]);
+        va[0] = facp * (vv[1] - vv[0]);
+      }
+      for (int k = 1; k < L - 1; ++k) {
+        const double facp = 0.5 * (1 / dp[k]) * e[k + 1];
+        const double facm = 0.5 * (1 / dp[k]) * e[k];
+        ta[k] = facp * (t[k + 1] - t[k]) + facm * (t[k] - t[k - 1]);
+        ua[k] = facp * (u[k + 1] - u[k]) + facm * (u[k] - u[k - 1]);
+        va[k] = facp * (vv[k + 1] - vv[k]) + facm * (vv[k] - vv[k - 1]);
+      }
+      {
+        const int k = L - 1;
+        const double facm = (0.5 * (1 / dp[k])) * e[k];
+        ta[k] = facm * (t[k] - t[k - 1]);
+        ua[k] = facm * (u[k] - u[k - 1]);
+        va[k] = facm * (vv[k] - vv[k - 1]);
+      }
+    }
+    double *ea = o->eta_dot_dpdn + (size_t)ie * E3;
+    for (size_t i = 0; i < E3; ++i) ea[i] += eta_ave_w * eta[i];
+  }
+  /* omega accumulation */
   for (size_t i = 0; i < E3; ++i) omega[i] += eta_ave_w * omb[i];
   /* temperature */
   gradient_sphere(o, ie, L, tn, b0, b1);
@@ -546,7 +590,7 @@ static void caar_element(Oracle *o, int ie, int nm1, int n0, int np1, int n0_qdp
     for (int k = 0; k < L; ++k) {
       const size_t i = (size_t)pt * L + k;
       const double vgrad_t = v0n[i] * b0[i] + v1n[i] * b1[i];
-      const double ttens = 0 - vgrad_t + KAPPA * tv[i] * omb[i];
+      const double ttens = (o->rsplit > 0 ? 0 : (PLANT(12) ? tva[i] : -tva[i])) - vgrad_t + KAPPA * tv[i] * omb[i]; /* plant 12 */
       double temp = ttens * dt + tnm1[i];
       temp *= sph[pt];
       tnp1[i] = temp;
@@ -569,9 +613,9 @@ static void caar_element(Oracle *o, int ie, int nm1, int n0, int np1, int n0_qdp
       if (PLANT(1)) vort[i] -= fcor[pt]; /* planted: Coriolis sign */
       else vort[i] += fcor[pt];
       eg0[i] *= -1;
-      eg0[i] += 0 + v1n[i] * vort[i];
+      eg0[i] += (o->rsplit > 0 ? 0 : -vva0[i]) + v1n[i] * vort[i];
       eg1[i] *= -1;
-      eg1[i] += 0 - v0n[i] * vort[i];
+      eg1[i] += (o->rsplit > 0 ? 0 : -vva1[i]) - v0n[i] * vort[i];
       eg0[i] *= dt;
       eg0[i] += v0m[i];
       eg1[i] *= dt;
@@ -580,21 +624,21 @@ static void caar_element(Oracle *o, int ie, int nm1, int n0, int np1, int n0_qdp
       v0p[i] = n0v;
       v1p[i] = n1v;
     }
-  /* dp3d (:468-493); eta_dot_dpdn buffer is zero when rsplit > 0 */
+  /* dp3d (:468-493); the eta_dot_dpdn buffer is zero when rsplit > 0 */
   double *dpp = DP(ie, np1);
   const double *dpm = DP(ie, nm1);
   for (int pt = 0; pt < NP2; ++pt)
     for (int k = 0; k < L; ++k) {
       const size_t i = (size_t)pt * L + k;
-      double tmp = 0;
+      double tmp = k + 1 < L ? eta[i + 1] : 0;
       tmp += divv[i];
-      tmp -= 0;
+      tmp -= eta[i];
       tmp = dpm[i] - tmp * dt;
       dpp[i] = sph[pt] * tmp;
     }
 }
 
-#define NWORK 16
+#define NWORK 18
 static double *work_for_thread(double *work, size_t e3) {
 #ifdef _OPENMP
   return work + (size_t)omp_get_thread_num() * NWORK * e3;
@@ -1137,9 +1181,10 @@ static void ppm_remap_column(int nlev, int remap_alg, const RemapGrid *g, double
   }
 }
 
-static int remap_run(Oracle *o, int np1, int np1_qdp) {
+static int remap_run(Oracle *o, int np1, int np1_qdp, double dt) {
   int bad = 0;
-  const int nf = 3 + o->qsize;
+  const int ns = o->rsplit > 0 ? 3 : 0; /* rsplit == 0: the states stay on their levels */
+  const int nf = ns + o->qsize;
 #pragma omp parallel for schedule(static) reduction(| : bad)
   for (int ie = 0; ie < o->nelem; ++ie) {
     double *dpn = DP(ie, np1), *ps = PSV(ie, np1);
@@ -1156,22 +1201,33 @@ static int remap_run(Oracle *o, int np1, int np1_qdp) {
     for (int pt = 0; pt < NP2; ++pt) {
       for (int k = 0; k < L; ++k) {
         tgt[k] = (o->hyai[k + 1] - o->hyai[k]) * o->ps0 + (o->hybi[k + 1] - o->hybi[k]) * ps[pt];
-        src[k] = dpn[pt * L + k];
+      }
+      for (int k = 0; k < L; ++k) {
+        if (ns) {
+          src[k] = dpn[pt * L + k];
+        } else {
+          /* _RemapFunctorRSplit<false>::compute_source_thickness (RemapFunctor.hpp:62-90) */
+          const double *e = o->eta_dot_dpdn + (size_t)ie * E3 + (size_t)pt * L;
+          const double en = k + 1 < L ? e[k + 1] : 0;
+          const double delta = en - e[k];
+          src[k] = tgt[k] + dt * delta;
+        }
         if (isnan(src[k]) || src[k] < 0.0) bad |= 1;
       }
       if (bad) continue;
       /* extrinsic states */
-      for (int k = 0; k < L; ++k) {
+      for (int k = 0; k < L && ns; ++k) {
         V(ie, np1, 0)[pt * L + k] *= src[k];
         V(ie, np1, 1)[pt * L + k] *= src[k];
         T(ie, np1)[pt * L + k] *= src[k];
       }
       ppm_grids(L, src, tgt, &g);
       for (int f = 0; f < nf; ++f) {
-        double *col = f == 0 ? V(ie, np1, 0) : f == 1 ? V(ie, np1, 1) : f == 2 ? T(ie, np1) : QDP(ie, np1_qdp, f - 3);
+        const int fs = f + 3 - ns;
+        double *col = fs == 0 ? V(ie, np1, 0) : fs == 1 ? V(ie, np1, 1) : fs == 2 ? T(ie, np1) : QDP(ie, np1_qdp, fs - 3);
         ppm_remap_column(L, o->remap_alg, &g, col + pt * L, 1, scr);
       }
-      for (int k = 0; k < L; ++k) {
+      for (int k = 0; k < L && ns; ++k) {
         V(ie, np1, 0)[pt * L + k] /= tgt[k];
         V(ie, np1, 1)[pt * L + k] /= tgt[k];
         T(ie, np1)[pt * L + k] /= tgt[k];
@@ -1274,7 +1330,7 @@ int oracle_set_params(Oracle *o, int remap_alg, int limiter_option, int rsplit, 
                       double nu_div, double nu_top, int hypervis_order, int hypervis_subcycle,
                       double hypervis_scaling, int ftype, int moist) {
   if (remap_alg < 1 || remap_alg > 3 || (limiter_option != 8 && limiter_option != 9) || time_step_type != 5 ||
-      hypervis_order != 2 || rsplit < 1 || qsplit < 1 || !(nu > 0) || !(nu_p > 0) || !(nu_div > 0) ||
+      hypervis_order != 2 || rsplit < 0 || qsplit < 1 || !(nu > 0) || !(nu_p > 0) || !(nu_div > 0) ||
       qsize != o->qsize || (ftype != -1 && ftype != 0 && ftype != 2))
     return 1;
   o->remap_alg = remap_alg; o->limiter_option = limiter_option; o->rsplit = rsplit; o->qsplit = qsplit;
@@ -1472,7 +1528,7 @@ int oracle_run_subcycle(Oracle *o, double dt, int *nstep, int *nm1, int *n0, int
     o->np1_qdp = 1 - o->n0_qdp;
   }
   const double dt_q = dt * o->qsplit;
-  const double dt_remap = dt_q * o->rsplit;
+  const double dt_remap = o->rsplit > 0 ? dt_q * o->rsplit : dt_q;
   if (o->ftype == 0) cam_forcing(o, dt_remap, 1);
   else if (o->ftype == 2) cam_forcing(o, dt_remap, 0);
   /* dp3d from ps */
@@ -1498,7 +1554,7 @@ int oracle_run_subcycle(Oracle *o, double dt, int *nstep, int *nm1, int *n0, int
     o->n0_qdp = it % 2 == 0 ? 0 : 1;
     o->np1_qdp = 1 - o->n0_qdp;
   }
-  int rc = remap_run(o, o->np1, o->np1_qdp) ? 1 : 0;
+  int rc = remap_run(o, o->np1, o->np1_qdp, dt_remap) ? 1 : 0;
   if (!rc) {
     /* update_q */
 #pragma omp parallel for schedule(static)

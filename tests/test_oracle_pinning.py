@@ -28,6 +28,11 @@ then fail.
   agrees with the contravariant form on a divergent field, both to a
   truncation error converging at second order.  Planted: undamping term
   dropped.
+* rsplit = 0 (Eulerian vertical advection, CaarFunctorImpl.hpp:237-300,
+  497-597, and the tracer-only remap from eta_dot_dpdn,
+  RemapFunctor.hpp:62-90): conserves dry and tracer mass, and agrees with
+  the vertically Lagrangian step (rsplit = 1) to within a tenth of what a
+  sign error in the vertical advection of T produces.  Planted: that sign.
 * CAM forcing (CamForcing.cpp): after a forced step the tracer mass (and,
   moist, the surface-pressure mass) equals the initial mass plus the clipped
   dt*FQ, since the step itself conserves mass; a uniform FT raises T by
@@ -340,3 +345,37 @@ def test_tensor_hyperviscosity_operators(oracle_lib):
 
 def test_tensor_checks_detect_planted_undamping_drop(oracle_lib):
     assert not tensor_ok(oracle.plant(11))
+
+
+# ---------------------------------------------------------------- rsplit = 0
+def eulerian_vs_lagrangian(L, ne=4):
+    def run(rs):
+        nu = 1e15
+        pr = build_problem(ne=ne, nlev=26, qsize=2, seed=0, nu=nu, nu_p=nu, nu_q=nu, nu_s=nu, nu_div=nu,
+                           nu_top=0.0, rsplit=rs)
+        sph = pr.metric["spheremp"]
+        s = oracle.OracleSession(pr, L if rs == 0 else oracle.lib())
+        nstep, nm1, n0, np1 = s.run(4)
+        out = s.state()
+        s.close()
+        dry = abs((sph * out["ps_v"][:, n0]).sum() / (sph * pr.state["ps_v"][:, 1]).sum() - 1)
+        q0 = np.einsum("epq,eclpq->c", sph, pr.state["Qdp"][:, 0, :2])
+        q1 = np.einsum("epq,eclpq->c", sph, out["Qdp"][:, nstep % 2, :2])
+        return out, dry, float(np.max(np.abs(q1 / q0 - 1)))
+    a, dry, qd = run(0)
+    b, _, _ = run(1)
+    n0 = 1  # both runs end on the same time levels
+    dT = float(np.abs(a["T"] - b["T"]).max())
+    dv = float(np.abs(a["v"] - b["v"]).max())
+    return dry, qd, dT, dv
+
+
+def test_rsplit0_conserves_mass_and_agrees_with_lagrangian(oracle_lib):
+    dry, qd, dT, dv = eulerian_vs_lagrangian(oracle_lib)
+    assert dry < 1e-14 and qd < 1e-13
+    assert dT < 0.02 and dv < 5e-3
+
+
+def test_rsplit0_check_detects_planted_vertical_advection_sign(oracle_lib):
+    _, _, dT, dv = eulerian_vs_lagrangian(oracle.plant(12))
+    assert not (dT < 0.02 and dv < 5e-3)
