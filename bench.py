@@ -5,7 +5,9 @@ perturbation, dt = 300 s, hyperviscosity subcycled 3x (nu = 1e15, nu_top =
 2.5e5), remap every step, limiter 8, PPM remap alg 1, no forcing.  One step =
 one prim_run_subcycle_c call through the C-ABI with all state resident in HBM.
 The same steps are then run in the mixed-precision form from the same state
-and its speed and per-field error against the FP64 run are reported.
+and its speed and per-field error against the FP64 run are reported, and the
+cost of an adjoint step against a forward step is measured on the ne16 L72
+Q4 differentiable-form workload (dry, rsplit = 0).
 
 Prints one JSON line: SYPD = dt / (365 * seconds per step), the roofline of the
 dominant kernel (HIP-event durations on the library stream over the timed
@@ -13,7 +15,7 @@ steps; algorithmic bytes from the kernel's array traffic), the whole-step
 fraction of HBM peak under the canonical bytes-per-step model, and a CPU
 baseline (the oracle, OpenMP over elements, one step of the same workload).
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-mixed]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-mixed] [--no-adjoint]
 """
 import argparse
 import json
@@ -135,6 +137,58 @@ def run_cpu(prob, min_seconds=10.0, max_steps=20):
     return (t1 - t0) / n, n, threads
 
 
+def run_adjoint(steps=10, traj_steps=48, every=8):
+    """Cost of the differentiable form on the ne16 L72 Q4 workload (dry,
+    rsplit = 0, dt = 300 s): one adjoint step (hommexx_hip_adjoint_step,
+    which recomputes the step's stage states and runs the reverse sweep)
+    against one forward step (prim_run_subcycle_c), both timed over
+    ``steps`` calls after a warm-up; and a checkpointed 48-step trajectory,
+    forward and backward through torch.autograd."""
+    import torch
+    from hommexx_amd.adjoint import (DiffSession, energy_mismatch, initial_state, region_weight,
+                                     synthetic_target, trajectory)
+    from hommexx_amd.problem import build_problem
+    nu = 7e15
+    prob = build_problem(ne=16, nlev=72, qsize=4, seed=32, rsplit=0, nu=nu, nu_p=nu, nu_q=nu, nu_s=nu, nu_div=nu)
+    progress("adjoint: ne16 L72 Q4 rsplit=0, nelem %d" % prob.nelem)
+    with DiffSession(prob) as s:
+        s.step(2)
+        s.h.synchronize()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.h.synchronize()
+        fwd = (time.perf_counter() - t0) / steps
+        st = s.get_state()
+        gen = torch.Generator(device="cuda").manual_seed(3)
+        lam = [torch.randn(a.shape, generator=gen, dtype=torch.float64, device="cuda") for a in st]
+        s.adjoint_step(st, lam)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.adjoint_step(st, lam)
+        torch.cuda.synchronize()
+        adj = (time.perf_counter() - t0) / steps
+        progress("adjoint: forward %.3f ms/step, adjoint %.3f ms/step" % (fwd * 1e3, adj * 1e3))
+        tgt, w = synthetic_target(prob, 6), region_weight(prob)
+        x = [a.clone().requires_grad_() for a in initial_state(prob)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J = energy_mismatch(prob, *trajectory(s, *x, traj_steps, every), tgt, w)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        J.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+    progress("adjoint: trajectory forward %.3f s, backward %.3f s" % (t1 - t0, t2 - t1))
+    return {"config": {"workload": "preqx step ne16 np4 L72 Q4, dt=300s, rsplit=0 (Eulerian vertical), dry, "
+                                   "hv subcycle 3, nu=7e15", "nelem": prob.nelem},
+            "forward_step_ms": round(fwd * 1e3, 3), "adjoint_step_ms": round(adj * 1e3, 3),
+            "adjoint_over_forward": round(adj / fwd, 3),
+            "trajectory": {"steps": traj_steps, "checkpoint_every": every,
+                           "forward_s": round(t1 - t0, 4), "backward_s": round(t2 - t1, 4),
+                           "backward_over_forward": round((t2 - t1) / (t1 - t0), 3)}}
+
+
 def copy_calibration(nbytes=1 << 30, reps=3):
     """A plain device copy of known size, for scaling the FETCH_SIZE /
     WRITE_SIZE counters: each launch reads and writes ``nbytes``."""
@@ -158,6 +212,7 @@ def main():
     ap.add_argument("--dt", type=float, default=300.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-cost leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -181,6 +236,7 @@ def main():
                  "speedup_vs_fp64": round(sec / msec, 4), "max_rel_l2_vs_fp64": max(err.values()),
                  "rel_l2_vs_fp64": err, "steps_compared": args.warmup + args.steps,
                  "dry_mass_rel_drift": mdry, "tracer_mass_rel_drift_max": mq}
+    adjoint = None if args.no_adjoint else run_adjoint()
     sypd = args.dt / (365.0 * sec)
     # dominant kernel by total time
     kern = {k: v for k, v in prof.items() if kernel_bytes(k, nelem, nlev, q, 1) is not None}
@@ -214,6 +270,7 @@ def main():
         "roofline": roof, "step_roofline": step_roof, "cpu_baseline": cpu,
         "conservation": {"dry_mass_rel_drift": dry_drift, "tracer_mass_rel_drift_max": q_drift},
         "mixed": mixed,
+        "adjoint": adjoint,
         "phases": phases,
     }
     print(json.dumps(out))

@@ -14,6 +14,8 @@
 //      Rgas*T_v*dp/(2p) -> phi
 //   C: phi = phis + 2*sum_below + own term (serial from the bottom)
 //   D: energy gradient, vorticity, v(np1)
+// With rsplit == 0 phases B and D also carry the Eulerian vertical advection
+// (eta_dot_dpdn from the column sums of div(v dp)).
 // Outputs go to pre-exchange buffers; the DSS kernel writes the state.
 #include "kernels.hpp"
 
@@ -29,7 +31,8 @@ __device__ __forceinline__ double virtual_t(const CaarArgs &a, const double *qv,
   return t * qt;
 }
 
-__global__ __launch_bounds__(256) void caar_kernel(CaarArgs a) {
+template <bool RS0>  // rsplit == 0: Eulerian vertical advection
+__global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
   const int pt = threadIdx.x & 15;
   const int chunk = threadIdx.x >> 6;
   const int ie = blockIdx.x * 4 + ((threadIdx.x >> 4) & 3);
@@ -72,6 +75,29 @@ __global__ __launch_bounds__(256) void caar_kernel(CaarArgs a) {
   }
   __syncthreads();
   // ---------------------------------------------------------------- B
+  // rsplit == 0: eta_dot_dpdn(k) = hybi(k) * S(L-1) - S(k-1), eta(0) = 0,
+  // with S the serial column sums of div(v dp) (:264-301)
+  double stotal = 0.0;
+  if (on && RS0)
+    for (int k = 0; k < L; ++k) stotal += a.divv[sb + (size_t)k * 16];
+  auto eta_of = [&](int k, double s_below) {  // s_below = S(k-1)
+    return k == 0 ? 0.0 : (k < L ? a.hybi[k] * stotal - s_below : 0.0);
+  };
+  // vertical advection of x at level k (preq_vertadv, :497-597)
+  auto vadv = [&](const double *x, size_t stride, int k, double dpk, double ek, double ek1) {
+    const double xk = x[(size_t)k * stride];
+    if (k == 0) {
+      const double facp = (0.5 * 1 / dpk) * ek1;
+      return facp * (x[(size_t)(k + 1) * stride] - xk);
+    }
+    if (k == L - 1) {
+      const double facm = (0.5 * (1 / dpk)) * ek;
+      return facm * (xk - x[(size_t)(k - 1) * stride]);
+    }
+    const double facp = 0.5 * (1 / dpk) * ek1;
+    const double facm = 0.5 * (1 / dpk) * ek;
+    return facp * (x[(size_t)(k + 1) * stride] - xk) + facm * (xk - x[(size_t)(k - 1) * stride]);
+  };
   if (on) {
     double integ = 0.0;
     for (int k = 0; k < k0; ++k) integ = integ + a.divv[sb + (size_t)k * 16];
@@ -86,6 +112,13 @@ __global__ __launch_bounds__(256) void caar_kernel(CaarArgs a) {
       p_prev = p;
       dp_prev = dp;
       const double divv = a.divv[s];
+      double ek = 0.0, ek1 = 0.0, tva = 0.0;
+      if (RS0) {
+        ek = eta_of(k, integ);
+        ek1 = eta_of(k + 1, integ + divv);
+        tva = vadv(a.t_n0 + sb, 16, k, dp, ek, ek1);
+        a.eta_acc[s] += eta * ek;
+      }
       double gp0, gp1;
       sph_grad(e, d, p, gp0, gp1);
       const double vgrad_p = u * gp0 + w * gp1;
@@ -95,13 +128,13 @@ __global__ __launch_bounds__(256) void caar_kernel(CaarArgs a) {
       double gt0, gt1;
       sph_grad(e, d, t, gt0, gt1);
       const double vgrad_t = u * gt0 + w * gt1;
-      const double ttens = 0 - vgrad_t + KAPPA * tv * om;
+      const double ttens = (RS0 ? -tva : 0) - vgrad_t + KAPPA * tv * om;
       double temp = ttens * dt + a.t_nm1[s];
       temp *= e.sph;
       a.out_t[s] = temp;
-      double tmp = 0;
+      double tmp = ek1;
       tmp += divv;
-      tmp -= 0;
+      tmp -= ek;
       tmp = a.dp_nm1[s] - tmp * dt;
       a.out_dp[s] = e.sph * tmp;
       a.phi[s] = RGAS * tv * (dp * 0.5 / p);
@@ -123,6 +156,9 @@ __global__ __launch_bounds__(256) void caar_kernel(CaarArgs a) {
   // ---------------------------------------------------------------- D
   if (on) {
     double p_prev = p_top, dp_prev = dp_top;
+    double integ = 0.0;
+    if (RS0)
+      for (int k = 0; k < k0; ++k) integ = integ + a.divv[sb + (size_t)k * 16];
     for (int k = k0; k < k1; ++k) {
       const size_t s = sb + (size_t)k * 16, v = vb + (size_t)k * 32;
       const double dp = a.dp_n0[s];
@@ -140,10 +176,18 @@ __global__ __launch_bounds__(256) void caar_kernel(CaarArgs a) {
       sph_grad_update(e, d, ephi, eg0, eg1);
       double vort = sph_vort(e, d, u, w);
       vort += e.fcor;
+      double va0 = 0.0, va1 = 0.0;
+      if (RS0) {
+        const double dv = a.divv[s];
+        const double ek = eta_of(k, integ), ek1 = eta_of(k + 1, integ + dv);
+        integ = integ + dv;
+        va0 = vadv(a.v_n0 + vb, 32, k, dp, ek, ek1);
+        va1 = vadv(a.v_n0 + vb + 16, 32, k, dp, ek, ek1);
+      }
       eg0 *= -1;
-      eg0 += 0 + w * vort;
+      eg0 += (RS0 ? -va0 : 0) + w * vort;
       eg1 *= -1;
-      eg1 += 0 - u * vort;
+      eg1 += (RS0 ? -va1 : 0) - u * vort;
       eg0 *= dt;
       eg0 += a.v_nm1[v];
       eg1 *= dt;
@@ -156,7 +200,8 @@ __global__ __launch_bounds__(256) void caar_kernel(CaarArgs a) {
 
 void launch_caar(hipStream_t st, const CaarArgs &a) {
   const int blocks = (a.nelem + 3) / 4;
-  hipLaunchKernelGGL(caar_kernel, dim3(blocks), dim3(64 * CNC), 0, st, a);
+  if (a.rsplit0) hipLaunchKernelGGL(caar_kernel<true>, dim3(blocks), dim3(64 * CNC), 0, st, a);
+  else hipLaunchKernelGGL(caar_kernel<false>, dim3(blocks), dim3(64 * CNC), 0, st, a);
 }
 
 // ------------------------------------------------------------------ small

@@ -81,6 +81,7 @@ struct Ctx {
   double *cv = nullptr, *ct = nullptr, *cdp = nullptr, *cdiv = nullptr;
   double *vt_a = nullptr, *tt_a = nullptr, *dpt_a = nullptr, *vt_b = nullptr, *tt_b = nullptr, *dpt_b = nullptr;
   double *qdp_pre = nullptr, *qbih = nullptr, *qbih_pre = nullptr, *qlim[2] = {}, *fdss_pre = nullptr;
+  double *qdp_pre2 = nullptr, *fdss_pre2 = nullptr;  // stage-2 pre-exchange buffers
   int *err_d = nullptr;
   int *err_h = nullptr;
   // time levels (0-based)
@@ -133,7 +134,10 @@ T *dalloc(size_t n) {
   void *p = nullptr;
   if (n == 0) n = 1;
   if (!hipok(hipMalloc(&p, n * sizeof(T)), "hipMalloc")) return nullptr;
+  // hipMemset runs on the null stream, which the library's non-blocking
+  // stream does not wait for: finish it before any kernel can touch p
   hipMemset(p, 0, n * sizeof(T));
+  if (!hipok(hipDeviceSynchronize(), "hipMemset")) return nullptr;
   C().allocs.push_back(p);
   return (T *)p;
 }
@@ -141,8 +145,11 @@ T *dalloc(size_t n) {
 void h2d(void *d, const void *h, size_t bytes) { hipok(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice), "h2d"); }
 void d2h(void *h, const void *d, size_t bytes) { hipok(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost), "d2h"); }
 
+void adj_reset();
+
 void free_all() {
   Ctx &c = C();
+  adj_reset();
   if (c.st) hipStreamSynchronize(c.st);
   for (void *p : c.allocs) hipFree(p);
   c.allocs.clear();
@@ -261,6 +268,8 @@ MetricPtrs metric() {
   m.metdet = c.metdet;
   m.phis = c.phis;
   m.dvv = c.dvv_d;
+  m.tensorvisc = c.tensorvisc;
+  m.vec_sph2cart = c.vec_sph2cart;
   return m;
 }
 
@@ -295,35 +304,46 @@ void alloc_work() {
   c.dpt_a = dalloc<double>(e3);
   c.dpt_b = dalloc<double>(e3);
   c.qdp_pre = dalloc<double>(q3);
+  c.qdp_pre2 = dalloc<double>(q3);
   c.qbih = dalloc<double>(q3);
   c.qbih_pre = dalloc<double>(q3);
   c.qlim[0] = dalloc<double>((size_t)c.nelem * (c.qsize ? c.qsize : 1) * 2 * c.nlev);
   c.qlim[1] = dalloc<double>((size_t)c.nelem * (c.qsize ? c.qsize : 1) * 2 * c.nlev);
   c.fdss_pre = dalloc<double>(e3);
+  c.fdss_pre2 = dalloc<double>(e3);
   c.err_d = dalloc<int>(1);
   if (!c.err_h) hipHostMalloc((void **)&c.err_h, sizeof(int));
   c.be_set = true;
 }
 
 // ------------------------------------------------------ step phases
-void caar_stage(int nm1, int n0, int np1, double dt, double eta_ave_w) {
+// one prognostic state: v [ie][lev][2][pt], T and dp3d [ie][lev][pt]
+struct StateBuf {
+  double *v, *t, *dp;
+};
+
+// out = DSS_r( sph*(a + dt*RHS(b)) ): one CAAR stage with its DSS
+void caar_run(const StateBuf &sa, const StateBuf &sb, const StateBuf &out, double dt, double eta_ave_w) {
   Ctx &c = C();
   CaarArgs a{};
   a.m = metric();
   a.nelem = c.nelem;
   a.nlev = c.nlev;
   a.qsize = c.qsize;
-  a.v_n0 = c.v[n0];
-  a.t_n0 = c.t[n0];
-  a.dp_n0 = c.dp[n0];
-  a.v_nm1 = c.v[nm1];
-  a.t_nm1 = c.t[nm1];
-  a.dp_nm1 = c.dp[nm1];
+  a.v_n0 = sb.v;
+  a.t_n0 = sb.t;
+  a.dp_n0 = sb.dp;
+  a.v_nm1 = sa.v;
+  a.t_nm1 = sa.t;
+  a.dp_nm1 = sa.dp;
   a.qdp_n0 = (c.p.moist && c.qsize > 0) ? c.qdp[c.n0_qdp] : nullptr;
   a.vn0 = c.vn0;
   a.omega = c.omega;
   a.phi = c.phi;
   a.divv = c.cdiv;
+  a.rsplit0 = c.p.rsplit == 0;
+  a.hybi = c.hybi_d;
+  a.eta_acc = c.eta_dot;
   a.out_v = c.cv;
   a.out_t = c.ct;
   a.out_dp = c.cdp;
@@ -341,12 +361,21 @@ void caar_stage(int nm1, int n0, int np1, double dt, double eta_ave_w) {
   x.src = c.dss_src;
   x.rsmp = c.rspheremp;
   const long long es = (long long)c.nlev * 16, ev = (long long)c.nlev * 32;
-  x.f[0] = {c.cv, c.v[np1], ev, 32};
-  x.f[1] = {c.cv + 16, c.v[np1] + 16, ev, 32};
-  x.f[2] = {c.ct, c.t[np1], es, 16};
-  x.f[3] = {c.cdp, c.dp[np1], es, 16};
+  x.f[0] = {c.cv, out.v, ev, 32};
+  x.f[1] = {c.cv + 16, out.v + 16, ev, 32};
+  x.f[2] = {c.ct, out.t, es, 16};
+  x.f[3] = {c.cdp, out.dp, es, 16};
   Scope s("caar_dss");
   launch_dss(c.st, x);
+}
+
+StateBuf level(int tl) {
+  Ctx &c = C();
+  return {c.v[tl], c.t[tl], c.dp[tl]};
+}
+
+void caar_stage(int nm1, int n0, int np1, double dt, double eta_ave_w) {
+  caar_run(level(nm1), level(n0), level(np1), dt, eta_ave_w);
 }
 
 bool stop_here(int phase) {
@@ -369,16 +398,16 @@ void u3_5stage(int nm1, int n0, int np1, double dt, double eta_ave_w) {
   caar_stage(nm1, np1, np1, 3.0 * dt / 4.0, 3.0 * eta_ave_w / 4.0);
 }
 
-void hyperviscosity(int np1, double dt, double eta_ave_w) {
+HvArgs hv_args(const StateBuf &x, double dt, double eta_ave_w) {
   Ctx &c = C();
   HvArgs a{};
   a.m = metric();
   a.src = c.dss_src;
   a.nelem = c.nelem;
   a.nlev = c.nlev;
-  a.v = c.v[np1];
-  a.t = c.t[np1];
-  a.dp = c.dp[np1];
+  a.v = x.v;
+  a.t = x.t;
+  a.dp = x.dp;
   a.vt_a = c.vt_a;
   a.tt_a = c.tt_a;
   a.dpt_a = c.dpt_a;
@@ -396,7 +425,15 @@ void hyperviscosity(int np1, double dt, double eta_ave_w) {
   a.nu_ratio1 = c.p.nu_ratio1;
   a.nu_ratio2 = c.p.nu_ratio2;
   a.nsub = c.p.hypervis_subcycle;
-  a.mixed = c.form == 1;
+  // the mixed form keeps the tensor form of hyperviscosity in FP64
+  a.consthv = c.consthv;
+  a.mixed = c.form == 1 && c.consthv;
+  return a;
+}
+
+void hyperviscosity(int np1, double dt, double eta_ave_w) {
+  Ctx &c = C();
+  const HvArgs a = hv_args(level(np1), dt, eta_ave_w);
   for (int cyc = 0; cyc < c.p.hypervis_subcycle; ++cyc) {
     Scope s("hv_cycle");
     launch_hv_cycle(c.st, a);
@@ -446,6 +483,8 @@ void advec_tracers(double dt) {
   a.nu_q = c.p.nu_q;
   a.limiter_option = c.p.limiter_option;
   a.mixed = c.form == 1;
+  a.bih32 = c.form == 1 && c.consthv;
+  a.consthv = c.consthv;
   a.rhs_viss = 0.0;
   const double dts = dt / 2.0;
   a.dt = dts;
@@ -475,13 +514,31 @@ void advec_tracers(double dt) {
     a.rhs_viss = stage == 2 ? 3.0 : 0.0;
     a.fdss = stage == 0 ? c.divdp_proj : stage == 1 ? c.eta_dot : c.omega;
     a.qdp_avg_n0 = stage == 2 ? c.qdp[c.n0_qdp] : nullptr;
-    {
-      Scope s("euler_advect");
-      launch_euler_advect(c.st, a, stage + 1);
-    }
-    {
+    // stage 2 gathers stage 1's exchange itself (the assembled stage-1 Qdp
+    // is never stored) unless the caller stops after stage 1
+    const bool fuse1 = c.debug_stop != 4;
+    if (stage == 1 && fuse1) {
+      EulerArgs b = a;
+      b.qdp_pre_in = c.qdp_pre;
+      b.fdss_pre_in = c.fdss_pre;
+      b.qdp_pre = c.qdp_pre2;
+      b.fdss_pre = c.fdss_pre2;
+      {
+        Scope s("euler_advect");
+        launch_euler_advect_gather(c.st, b);
+      }
+      b.fdss = c.eta_dot;
       Scope s("euler_dss");
-      launch_euler_dss(c.st, a, stage == 2);
+      launch_euler_dss(c.st, b, 0);
+    } else {
+      {
+        Scope s("euler_advect");
+        launch_euler_advect(c.st, a, stage + 1);
+      }
+      if (!(stage == 0 && fuse1)) {
+        Scope s("euler_dss");
+        launch_euler_dss(c.st, a, stage == 2);
+      }
     }
     if (stage == 0 && stop_here(4)) return;
     if (stage == 1) {
@@ -525,11 +582,14 @@ void prim_step(double dt) {
 
 void remap(double dt_remap) {
   Ctx &c = C();
-  (void)dt_remap;
   RemapArgs a{};
   a.nelem = c.nelem;
   a.nlev = c.nlev;
   a.qsize = c.qsize;
+  a.nstates = c.p.rsplit > 0 ? 3 : 0;
+  a.eta = c.eta_dot;
+  a.src = c.cdiv;  // CAAR's scratch is free at the remap
+  a.dt = dt_remap;
   a.remap_alg = c.p.remap_alg;
   a.v = c.v[c.np1];
   a.t = c.t[c.np1];
@@ -544,6 +604,170 @@ void remap(double dt_remap) {
   a.error = c.err_d;
   Scope s("remap");
   launch_remap(c.st, a);
+}
+
+// ------------------------------------------------------------ adjoint
+// Reverse-mode derivative of one dry rsplit = 0 step (no tracers on the
+// dynamics path).  The forward stage states are recomputed into private
+// buffers first; the step's model state is not touched.
+struct AdjBufs {
+  size_t f3 = 0;  // doubles in one 3-D scalar
+  int nsub = 0;
+  std::vector<StateBuf> U, H;  // CAAR stage inputs U0..U4, HV cycle states H0..Hnsub
+  StateBuf W{}, LX{}, L0{}, L1{}, LP{}, S{}, M{};
+  double *div, *p, *cc, *stot, *pb, *phib, *etab, *ombp, *divb, *lv, *lt, *ldp;
+  double *base = nullptr;
+};
+AdjBufs *adjb = nullptr;
+
+void adj_reset() {
+  delete adjb;
+  adjb = nullptr;
+}
+
+StateBuf carve_state(double *&q, size_t f3) {
+  StateBuf s{q, q + 2 * f3, q + 3 * f3};
+  q += 4 * f3;
+  return s;
+}
+
+bool adj_alloc() {
+  Ctx &c = C();
+  const size_t f3 = (size_t)c.nelem * c.nlev * 16;
+  const int nsub = c.p.hypervis_subcycle;
+  if (adjb && adjb->f3 == f3 && adjb->nsub == nsub) return true;
+  if (!adjb) adjb = new AdjBufs();
+  AdjBufs &b = *adjb;
+  const int nstates = 5 + 1 + (nsub + 1) + 6;  // U, W, H, LX L0 L1 LP S M
+  const size_t total = (size_t)nstates * 4 * f3 + 12 * f3 + (size_t)c.nelem * 16;
+  b.base = dalloc<double>(total);  // owned by the session (freed with it)
+  if (!b.base) return false;
+  b.f3 = f3;
+  b.nsub = nsub;
+  double *q = b.base;
+  b.U.clear();
+  b.H.clear();
+  for (int i = 0; i < 5; ++i) b.U.push_back(carve_state(q, f3));
+  b.W = carve_state(q, f3);
+  for (int i = 0; i <= nsub; ++i) b.H.push_back(carve_state(q, f3));
+  b.LX = carve_state(q, f3);
+  b.L0 = carve_state(q, f3);
+  b.L1 = carve_state(q, f3);
+  b.LP = carve_state(q, f3);
+  b.S = carve_state(q, f3);
+  b.M = carve_state(q, f3);
+  double **sc[] = {&b.div, &b.p, &b.cc, &b.pb, &b.phib, &b.etab, &b.ombp, &b.divb, &b.lt, &b.ldp};
+  for (double **x : sc) {
+    *x = q;
+    q += f3;
+  }
+  b.lv = q;
+  q += 2 * f3;
+  b.stot = q;
+  return true;
+}
+
+void copy_state(const StateBuf &dst, const StateBuf &src, size_t f3) {
+  Ctx &c = C();
+  hipMemcpyAsync(dst.v, src.v, 4 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+}
+
+// y = DSS_r(x) for the four fields of a state
+void dss_state(const StateBuf &x, const StateBuf &y) {
+  Ctx &c = C();
+  DssArgs a{};
+  a.nf = 4;
+  a.nelem = c.nelem;
+  a.nlev = c.nlev;
+  a.src = c.dss_src;
+  a.rsmp = c.rspheremp;
+  const long long es = (long long)c.nlev * 16, ev = (long long)c.nlev * 32;
+  a.f[0] = {x.v, y.v, ev, 32};
+  a.f[1] = {x.v + 16, y.v + 16, ev, 32};
+  a.f[2] = {x.t, y.t, es, 16};
+  a.f[3] = {x.dp, y.dp, es, 16};
+  launch_dss(c.st, a);
+}
+
+// adjoint of out = DSS_r(sph*(a + h*RHS(b))), lam = adjoint of out
+void caar_adjoint(const StateBuf &xb, const StateBuf &lam, double h, const StateBuf &lb, bool b_acc,
+                  const StateBuf &la1, double c1, const StateBuf *la2, double c2) {
+  Ctx &c = C();
+  AdjBufs &b = *adjb;
+  dss_state(lam, b.LP);
+  AdjCaarArgs a{};
+  a.m = metric();
+  a.nelem = c.nelem;
+  a.nlev = c.nlev;
+  a.v = xb.v;
+  a.t = xb.t;
+  a.dp = xb.dp;
+  a.lp_v = b.LP.v;
+  a.lp_t = b.LP.t;
+  a.lp_dp = b.LP.dp;
+  a.hybi = c.hybi_d;
+  a.h = h;
+  a.p_top = c.hyai0 * c.ps0;
+  a.div = b.div;
+  a.p = b.p;
+  a.c = b.cc;
+  a.stot = b.stot;
+  a.pb = b.pb;
+  a.phib = b.phib;
+  a.etab = b.etab;
+  a.ombp = b.ombp;
+  a.divb = b.divb;
+  a.lv = b.lv;
+  a.lt = b.lt;
+  a.ldp = b.ldp;
+  a.b_v = lb.v;
+  a.b_t = lb.t;
+  a.b_dp = lb.dp;
+  a.b_acc = b_acc;
+  a.a1_v = la1.v;
+  a.a1_t = la1.t;
+  a.a1_dp = la1.dp;
+  a.c1 = c1;
+  if (la2) {
+    a.a2_v = la2->v;
+    a.a2_t = la2->t;
+    a.a2_dp = la2->dp;
+    a.c2 = c2;
+  }
+  launch_adj_caar(c.st, a);
+}
+
+// adjoint of one hyperviscosity cycle x -> xo; lam (in place)
+void hv_adjoint(const StateBuf &x, const StateBuf &xo, const StateBuf &lam, double dt_cycle) {
+  Ctx &c = C();
+  AdjBufs &b = *adjb;
+  AdjHvArgs a{};
+  a.m = metric();
+  a.nelem = c.nelem;
+  a.nlev = c.nlev;
+  a.x_v = x.v;
+  a.xo_v = xo.v;
+  a.l_v = lam.v;
+  a.l_t = lam.t;
+  a.l_dp = lam.dp;
+  a.s_v = b.S.v;
+  a.s_t = b.S.t;
+  a.s_dp = b.S.dp;
+  a.m_v = b.M.v;
+  a.m_t = b.M.t;
+  a.m_dp = b.M.dp;
+  a.dt = dt_cycle;
+  a.nu = c.p.nu;
+  a.nu_s = c.p.nu_s;
+  a.nu_p = c.p.nu_p;
+  a.nu_top = c.p.nu_top;
+  a.nu_ratio1 = c.p.nu_ratio1;
+  a.nu_ratio2 = c.p.nu_ratio2;
+  launch_adj_hv_pre(c.st, a);
+  dss_state(b.S, b.M);
+  launch_adj_hv_a(c.st, a);
+  dss_state(b.S, b.M);
+  launch_adj_hv_b(c.st, a);
 }
 
 // Fortran (np,np,nlev,...) blocks are already [lev][pt] per element; only the
@@ -710,8 +934,7 @@ void init_simulation_params_c(const int &remap_alg, const int &limiter_option, c
   if (!(nu_p > 0)) return snprintf(buf, 64, "%g", nu_p), bad("nu_p", buf);
   if (!(nu > 0)) return snprintf(buf, 64, "%g", nu), bad("nu", buf);
   if (!(nu_div > 0)) return snprintf(buf, 64, "%g", nu_div), bad("nu_div", buf);
-  if (hypervis_scaling != 0.0) return snprintf(buf, 64, "%g", hypervis_scaling), bad("hypervis_scaling", buf);
-  if (rsplit < 1) return snprintf(buf, 64, "%d", rsplit), bad("rsplit", buf);
+  if (rsplit < 0) return snprintf(buf, 64, "%d", rsplit), bad("rsplit", buf);
   if (qsplit < 1 || hypervis_subcycle < 1 || qsize < 0)
     return bad("qsplit/hypervis_subcycle/qsize", "out of range");
   Params &p = c.p;
@@ -1077,6 +1300,95 @@ int hommexx_hip_profile_names(char *buf, int len) {
     buf[len - 1] = 0;
   }
   return (int)s.size() + 1;
+}
+
+// ---------------------------------------------------------- differentiable form
+// Device copies of the prognostic state at time level n0, in the library's
+// layouts: v [ie][lev][2][pt], T [ie][lev][pt], ps [ie][pt].  Used by the
+// checkpointing harness.
+int hommexx_hip_get_state(double *v, double *T, double *ps) {
+  Ctx &c = C();
+  if (!c.states_set) {
+    fail(55, "hommexx_hip_get_state: no state");
+    return 1;
+  }
+  const size_t f3 = (size_t)c.nelem * c.nlev * 16;
+  hipMemcpyAsync(v, c.v[c.n0], 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  hipMemcpyAsync(T, c.t[c.n0], f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  hipMemcpyAsync(ps, c.ps[c.n0], (size_t)c.nelem * 16 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  return hipok(hipStreamSynchronize(c.st), "hommexx_hip_get_state") ? 0 : 2;
+}
+
+int hommexx_hip_set_state(const double *v, const double *T, const double *ps) {
+  Ctx &c = C();
+  if (!c.states_set || !c.hv_set) {
+    fail(55, "hommexx_hip_set_state: no state");
+    return 1;
+  }
+  const size_t f3 = (size_t)c.nelem * c.nlev * 16;
+  hipMemcpyAsync(c.v[c.n0], v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  hipMemcpyAsync(c.t[c.n0], T, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  hipMemcpyAsync(c.ps[c.n0], ps, (size_t)c.nelem * 16 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  launch_dp3d_from_ps(c.st, c.nelem, c.nlev, c.hyai_d, c.hybi_d, c.ps0, c.ps[c.n0], c.dp[c.n0]);
+  return hipok(hipStreamSynchronize(c.st), "hommexx_hip_set_state") ? 0 : 2;
+}
+
+// Adjoint of one step of length dt (one prim_run_subcycle_c call) of the dry
+// rsplit = 0 configuration, taken at the step's input state (v, T, ps):
+// on entry lam_* hold the adjoint of the step's output (v, T, ps), on return
+// the adjoint of its input.  All pointers are device memory in the layouts of
+// hommexx_hip_get_state.  The model state of the session is not changed.
+int hommexx_hip_adjoint_step(double dt, const double *v, const double *T, const double *ps, double *lam_v,
+                             double *lam_T, double *lam_ps) {
+  Ctx &c = C();
+  if (!ready("hommexx_hip_adjoint_step")) return 1;
+  if (c.p.rsplit != 0 || c.p.qsplit != 1 || c.p.moist || !c.consthv || c.form != 0 || c.p.ftype != -1) {
+    fail(70, "hommexx_hip_adjoint_step: needs rsplit = 0, qsplit = 1, no moisture, constant-coefficient "
+             "hyperviscosity, no forcing and the FP64 form");
+    return 2;
+  }
+  if (!c.be_set) init_boundary_exchanges_c();
+  if (!adj_alloc()) return 3;
+  AdjBufs &b = *adjb;
+  const size_t f3 = b.f3;
+  const int nsub = c.p.hypervis_subcycle;
+  std::vector<StateBuf> &U = b.U, &H = b.H;
+  // forward: stage inputs and hyperviscosity states
+  {
+    Scope s("adj_forward");
+    hipMemcpyAsync(U[0].v, v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+    hipMemcpyAsync(U[0].t, T, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+    launch_dp3d_from_ps(c.st, c.nelem, c.nlev, c.hyai_d, c.hybi_d, c.ps0, ps, U[0].dp);
+    caar_run(U[0], U[0], U[1], dt / 5.0, 0.0);
+    caar_run(U[0], U[1], U[2], dt / 5.0, 0.0);
+    caar_run(U[0], U[2], U[3], dt / 3.0, 0.0);
+    caar_run(U[0], U[3], U[4], 2.0 * dt / 3.0, 0.0);
+    copy_state(b.W, U[1], f3);
+    launch_rk_combo(c.st, c.nelem, c.nlev, b.W.v, b.W.t, b.W.dp, U[0].v, U[0].t, U[0].dp);
+    caar_run(b.W, U[4], H[0], 3.0 * dt / 4.0, 0.0);
+    for (int cyc = 0; cyc < nsub; ++cyc) {
+      copy_state(H[cyc + 1], H[cyc], f3);
+      launch_hv_cycle(c.st, hv_args(H[cyc + 1], dt, 0.0));
+    }
+  }
+  // reverse sweep
+  Scope s("adj_reverse");
+  hipMemcpyAsync(b.LX.v, lam_v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  hipMemcpyAsync(b.LX.t, lam_T, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  launch_adj_fill_dp(c.st, c.nelem, c.nlev, lam_ps, b.LX.dp);  // ps = hyai(0) ps0 + sum_k dp3d
+  for (int cyc = nsub - 1; cyc >= 0; --cyc) hv_adjoint(H[cyc], H[cyc + 1], b.LX, dt / nsub);
+  launch_adj_zero(c.st, 4 * f3, b.L0.v);
+  launch_adj_zero(c.st, 4 * f3, b.L1.v);
+  // stage 5 from w = (5 u1 - u0)/4 and u4; then stages 4..1
+  caar_adjoint(U[4], b.LX, 3.0 * dt / 4.0, b.LX, false, b.L1, 1.25, &b.L0, -0.25);
+  caar_adjoint(U[3], b.LX, 2.0 * dt / 3.0, b.LX, false, b.L0, 1.0, nullptr, 0.0);
+  caar_adjoint(U[2], b.LX, dt / 3.0, b.LX, false, b.L0, 1.0, nullptr, 0.0);
+  caar_adjoint(U[1], b.LX, dt / 5.0, b.L1, true, b.L0, 1.0, nullptr, 0.0);
+  caar_adjoint(U[0], b.L1, dt / 5.0, b.L0, true, b.L0, 1.0, nullptr, 0.0);
+  hipMemcpyAsync(lam_v, b.L0.v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  hipMemcpyAsync(lam_T, b.L0.t, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  launch_adj_ps(c.st, c.nelem, c.nlev, c.hybi_d, b.L0.dp, lam_ps);  // dp3d = dhyai ps0 + dhybi ps
+  return hipok(hipStreamSynchronize(c.st), "hommexx_hip_adjoint_step") ? 0 : 4;
 }
 
 // Current time-level indices (0-based) and sizes.

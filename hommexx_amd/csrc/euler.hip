@@ -94,6 +94,19 @@ __global__ __launch_bounds__(256) void euler_nbr_minmax_kernel(EulerArgs a) {
   a.qlim_new[own + L] = mx;
 }
 
+// gather of point (ie, pt) at level k from a levels-fastest field with
+// element stride es (own value, then the sources in the reference's order)
+// (FP32 buffers of the mixed form are accumulated in FP64 the same way)
+template <typename T>
+__device__ __forceinline__ double gather_lf(const T *f, size_t es, size_t off, int L, int ie, int pt, int k, int s0,
+                                            int s1, int s2) {
+  double x = f[(size_t)ie * es + off + (size_t)pt * L + k];
+  if (s0 >= 0) x += (double)f[(size_t)(s0 >> 4) * es + off + (size_t)(s0 & 15) * L + k];
+  if (s1 >= 0) x += (double)f[(size_t)(s1 >> 4) * es + off + (size_t)(s1 & 15) * L + k];
+  if (s2 >= 0) x += (double)f[(size_t)(s2 >> 4) * es + off + (size_t)(s2 & 15) * L + k];
+  return x;
+}
+
 // ------------------------------------------------------------ limiters
 // limiter_optim_iter_full (:767-823), one slab = one level of one tracer
 __device__ __forceinline__ void limit8_slab(double mass, double minp, double maxp, double &x, double c) {
@@ -151,19 +164,66 @@ __device__ __forceinline__ void limit9_slab(double minp, double maxp, double &x,
 // Setup phase (:585-626) + tracer phase (:571-687) for every tracer of an
 // (element, level) slab.  At stage 2 the qmin/qmax update of that stage
 // (min/max with the previous bounds, no neighbour exchange) runs first.
-template <bool MIXED>
+//
+// GATHER (stage 2 only): the stage-1 exchange is done here instead of by a
+// separate kernel.  A block is one element x 8 levels (128 lanes); lanes
+// (pt, kk) gather the stage-1 pre-exchange values (levels fastest) of point
+// pt at level k0+kk with the reference's accumulation order, an LDS tile
+// hands them to the slab lanes (kk, pt), and the assembled stage-1 Qdp is
+// used directly (never stored).  The assembled extra field (divdp_proj) is
+// stored, later stages read it.
+constexpr int AGL = 8;
+
+template <bool MIXED, bool GATHER>
 __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stage) {
+  __shared__ double tile[GATHER ? 2 : 1][AGL][17];
   size_t slab;
   int ie, k, pt;
-  if (!slab_ie_k((size_t)a.nelem * a.nlev, a.nlev, slab, ie, k, pt)) return;
   const int L = a.nlev, Q = a.qsize;
+  // gather role (GATHER only)
+  int gpt = 0, gk = 0, gkk = 0, kk = 0, g0 = -1, g1 = -1, g2 = -1;
+  bool gon = false, son = true;
+  double gr = 0.0;
+  if (GATHER) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    ie = lb % a.nelem;
+    const int k0 = (lb / a.nelem) * AGL;
+    if (k0 >= L) return;  // whole block
+    kk = threadIdx.x >> 4;
+    pt = threadIdx.x & 15;
+    k = k0 + kk;
+    son = k < L;
+    gpt = threadIdx.x >> 3;
+    gkk = threadIdx.x & 7;
+    gk = k0 + gkk;
+    gon = gk < L;
+    const int *src = a.src + ((size_t)ie * 16 + gpt) * 3;
+    g0 = src[0];
+    g1 = src[1];
+    g2 = src[2];
+    gr = a.m.rspheremp[(size_t)ie * 16 + gpt];
+    slab = (size_t)ie * L + (son ? k : 0);
+  } else {
+    if (!slab_ie_k((size_t)a.nelem * a.nlev, a.nlev, slab, ie, k, pt)) return;
+  }
   ElemLane e;
   load_elem_light(a.m, ie, pt, e);
   DvvLane d;
   load_dvv(a.m.dvv, pt, d);
   const size_t s = slab * 16 + pt, v = slab * 32 + pt;
   const double rhsmdt = a.rhs_multiplier * a.dt;
-  const double dp = a.derived_dp[s] - rhsmdt * a.divdp_proj[s];
+  double dpproj;
+  if (GATHER) {
+    // assembled divdp_proj of stage 1
+    if (gon) tile[0][gkk][gpt] = gather_lf(a.fdss_pre_in, (size_t)16 * L, 0, L, ie, gpt, gk, g0, g1, g2) * gr;
+    __syncthreads();
+    dpproj = tile[0][kk][pt];
+    if (son) a.divdp_proj_w[s] = dpproj;
+    __syncthreads();
+  } else {
+    dpproj = a.divdp_proj[s];
+  }
+  const double dp = a.derived_dp[s] - rhsmdt * dpproj;
   const double vs0 = a.vn0[v] / dp, vs1 = a.vn0[v + 16] / dp;
   double dpdissk = dp - a.dt * a.divdp[s];
   const bool add_ps_diss = a.nu_p > 0 && a.rhs_viss != 0.0;
@@ -171,7 +231,7 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
     const double diss_fac = -a.rhs_viss * a.dt * a.nu_q;
     dpdissk += diss_fac * a.dpdiss_bih[s] / e.sph;
   }
-  a.fdss_pre[((size_t)ie * 16 + pt) * L + k] = a.fdss[s] * e.sph;  // [ie][pt][lev]
+  if (son) a.fdss_pre[((size_t)ie * 16 + pt) * L + k] = a.fdss[s] * e.sph;  // [ie][pt][lev]
   const bool add_hv = a.rhs_viss != 0.0;
   const bool limit = a.limiter_option == 8 || a.limiter_option == 9;
   DvvLaneF df;
@@ -182,11 +242,25 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
   }
   const double c = e.sph * dpdissk;
   const double s1 = limit ? slab_sum_serial(c) : 0.0;  // same for every tracer
+  const size_t es = (size_t)Q * 16 * L;
+  double gnext = 0.0;
+  if (GATHER && gon && Q > 0) gnext = gather_lf(a.qdp_pre_in, es, 0, L, ie, gpt, gk, g0, g1, g2);
+  const int kc = son ? k : L - 1;  // reads of a padding slab stay in bounds
   for (int q = 0; q < Q; ++q) {
-    const size_t qi = (((size_t)ie * Q + q) * L + k) * 16 + pt;
-    const double qd = a.qdp_n0[qi];
+    const size_t qi = (((size_t)ie * Q + q) * L + kc) * 16 + pt;
+    double qd;
+    if (GATHER) {
+      // the stage-1 exchange of tracer q; tracer q+1's gathers are issued
+      // before this tracer's arithmetic
+      if (gon) tile[q & 1][gkk][gpt] = gnext * gr;
+      if (gon && q + 1 < Q) gnext = gather_lf(a.qdp_pre_in, es, (size_t)(q + 1) * 16 * L, L, ie, gpt, gk, g0, g1, g2);
+      __syncthreads();
+      qd = tile[q & 1][kk][pt];
+    } else {
+      qd = a.qdp_n0[qi];
+    }
     double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
-    double minp = ql[k], maxp = ql[L + k];
+    double minp = ql[kc], maxp = ql[L + kc];
     if (stage == 2) {  // compute_qmin_qmax with rhs_multiplier == 1
       const double xq = qd / dp;
       minp = dmin(minp, slab_min(xq));
@@ -196,14 +270,14 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
     if (MIXED) {
       // flux divergence in FP32 with compensated contractions; the update
       // qdp - dt*div + qbih stays FP64
-      const double q0 = add_hv ? (double)((const float *)a.qbih)[qi] : 0;
+      const double q0 = add_hv ? (a.bih32 ? (double)((const float *)a.qbih)[qi] : a.qbih[qi]) : 0;
       const double fa = vs0 * qd, fb = vs1 * qd;
       const float gv0 = (float)((e.Dinv[0] * fa + e.Dinv[2] * fb) * e.metdet);
       const float gv1 = (float)((e.Dinv[1] * fa + e.Dinv[3] * fb) * e.metdet);
       const float dv = (row_contractf(df.J, gv0) + col_contractf(df.I, gv1)) * rmd;
       ptens = (qd + -a.dt * (double)dv + q0);
     } else {
-      const double q0 = add_hv ? a.qbih[qi] : 0;
+      const double q0 = add_hv ? (a.bih32 ? (double)((const float *)a.qbih)[qi] : a.qbih[qi]) : 0;
       ptens = sph_div_update(e, d, -a.dt, vs0, vs1, qd, q0);
     }
     bool wmin = stage == 2, wmax = stage == 2;
@@ -228,11 +302,11 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
         ptens = x * dpdissk;
       }
     }
-    if (pt == 0) {
+    if (pt == 0 && son) {
       if (wmin) ql[k] = minp;
       if (wmax) ql[L + k] = maxp;
     }
-    a.qdp_pre[(((size_t)ie * Q + q) * 16 + pt) * L + k] = e.sph * ptens;  // [ie][q][pt][lev]
+    if (son) a.qdp_pre[(((size_t)ie * Q + q) * 16 + pt) * L + k] = e.sph * ptens;  // [ie][q][pt][lev]
   }
 }
 
@@ -271,18 +345,6 @@ __device__ __forceinline__ bool xchg_lane(int nelem, int nlev, XLane &x) {
   return true;
 }
 
-// gather of point (ie, pt) at level k from a levels-fastest field with
-// element stride es (own value, then the sources in the reference's order)
-// (FP32 buffers of the mixed form are accumulated in FP64 the same way)
-template <typename T>
-__device__ __forceinline__ double gather_lf(const T *f, size_t es, size_t off, int L, int ie, int pt, int k, int s0,
-                                            int s1, int s2) {
-  double x = f[(size_t)ie * es + off + (size_t)pt * L + k];
-  if (s0 >= 0) x += (double)f[(size_t)(s0 >> 4) * es + off + (size_t)(s0 & 15) * L + k];
-  if (s1 >= 0) x += (double)f[(size_t)(s1 >> 4) * es + off + (size_t)(s1 & 15) * L + k];
-  if (s2 >= 0) x += (double)f[(size_t)(s2 >> 4) * es + off + (size_t)(s2 & 15) * L + k];
-  return x;
-}
 
 // DSS of tracers [QB*blockIdx.y, QB*blockIdx.y + QB) and, in the first batch,
 // of the extra field.  avg: stage 3's SSP average (Qdp(n0) + 2 Qdp(np1)) / 3
@@ -368,7 +430,7 @@ __global__ __launch_bounds__(256) void euler_qmm_lap_kernel(EulerArgs a) {
 // rspheremp), second laplacian, times -rhs_viss*dt*nu_q*dp0/spheremp, for
 // tracers [QB*blockIdx.y, QB*blockIdx.y + QB).  After the LDS transpose the
 // 16-lane groups are slabs (one level, 16 points) for the laplacian.
-template <bool MIXED>
+template <bool MIXED, bool TENSOR>
 __global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
   __shared__ double tile[2][XL][17];
   XLane x;
@@ -398,6 +460,8 @@ __global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
     load_elemf(a.m, x.ie, x.wpt, ef);
     load_dvvf(a.m.dvv, x.wpt, df);
   }
+  double tv[4], vs[6];
+  if (TENSOR) load_tensor(a.m, x.ie, x.wpt, tv, vs);
   const double f = -a.rhs_viss * a.dt * a.nu_q;
   const double dp0 = wkv ? a.dp0[wk] : 0.0;
 #pragma unroll
@@ -412,6 +476,10 @@ __global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
     if (MIXED) {
       const double lap = sph_laplacef(ef, df, (float)y);
       if (wkv) ((float *)a.qbih)[o] = (float)(f * dp0 * lap / e.sph);
+    } else if (TENSOR) {
+      // BIHPostTensorHV (EulerStepFunctorImpl.hpp:281-291)
+      const double lap = sph_laplace_tensor(e, d, tv, y);
+      if (wkv) a.qbih[o] = (f * dp0 * lap / e.sph);
     } else {
       const double lap = sph_laplace(e, d, y);
       if (wkv) a.qbih[o] = (f * dp0 * lap / e.sph);
@@ -431,8 +499,14 @@ void launch_euler_nbr_minmax(hipStream_t st, const EulerArgs &a) {
 }
 
 void launch_euler_advect(hipStream_t st, const EulerArgs &a, int stage) {
-  if (a.mixed) hipLaunchKernelGGL(euler_advect_kernel<true>, dim3(slab_blocks(a)), dim3(256), 0, st, a, stage);
-  else hipLaunchKernelGGL(euler_advect_kernel<false>, dim3(slab_blocks(a)), dim3(256), 0, st, a, stage);
+  if (a.mixed) hipLaunchKernelGGL((euler_advect_kernel<true, false>), dim3(slab_blocks(a)), dim3(256), 0, st, a, stage);
+  else hipLaunchKernelGGL((euler_advect_kernel<false, false>), dim3(slab_blocks(a)), dim3(256), 0, st, a, stage);
+}
+
+void launch_euler_advect_gather(hipStream_t st, const EulerArgs &a) {
+  const int nb = ((a.nelem * ((a.nlev + AGL - 1) / AGL)) + 7) & ~7;
+  if (a.mixed) hipLaunchKernelGGL((euler_advect_kernel<true, true>), dim3(nb), dim3(16 * AGL), 0, st, a, 2);
+  else hipLaunchKernelGGL((euler_advect_kernel<false, true>), dim3(nb), dim3(16 * AGL), 0, st, a, 2);
 }
 
 void launch_euler_dss(hipStream_t st, const EulerArgs &a, int avg) {
@@ -441,15 +515,16 @@ void launch_euler_dss(hipStream_t st, const EulerArgs &a, int avg) {
 }
 
 void launch_euler_qmm_lap(hipStream_t st, const EulerArgs &a) {
-  if (a.mixed) hipLaunchKernelGGL(euler_qmm_lap_kernel<true>, dim3(slab_blocks(a)), dim3(256), 0, st, a);
+  if (a.bih32) hipLaunchKernelGGL(euler_qmm_lap_kernel<true>, dim3(slab_blocks(a)), dim3(256), 0, st, a);
   else hipLaunchKernelGGL(euler_qmm_lap_kernel<false>, dim3(slab_blocks(a)), dim3(256), 0, st, a);
 }
 
 void launch_euler_bih_post(hipStream_t st, const EulerArgs &a) {
   if (a.qsize <= 0) return;
   const dim3 g(xchg_blocks_x(a.nelem, a.nlev), (a.qsize + QB - 1) / QB);
-  if (a.mixed) hipLaunchKernelGGL(euler_bih_post_kernel<true>, g, dim3(256), 0, st, a);
-  else hipLaunchKernelGGL(euler_bih_post_kernel<false>, g, dim3(256), 0, st, a);
+  if (a.bih32) hipLaunchKernelGGL((euler_bih_post_kernel<true, false>), g, dim3(256), 0, st, a);
+  else if (a.consthv) hipLaunchKernelGGL((euler_bih_post_kernel<false, false>), g, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL((euler_bih_post_kernel<false, true>), g, dim3(256), 0, st, a);
 }
 
 }  // namespace hx

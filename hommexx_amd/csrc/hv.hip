@@ -5,8 +5,8 @@
 //   DSS(*rsmp)
 //   lap2+pre  : second laplacians, dpdiss accumulators, -nu scaling, the
 //               nu_top sponge on the top three levels, dptens <- dp3d*sph + dt*dptens
-//   DSS
-//   update    : apply the inverse mass, v += , T += - heating/cp, dp3d = dptens*rsmp
+//   DSS + update : assemble, apply the inverse mass, v += , T += - heating/cp,
+//                  dp3d = dptens*rsmp
 // The mixed form swaps in FP32 laplacian kernels (below) and an FP32 first
 // exchange; the second exchange and the update are the same.
 #include "kernels.hpp"
@@ -40,6 +40,7 @@ __global__ __launch_bounds__(256) void hv_lap1_kernel(HvArgs a) {
   a.vt_a[v + 16] = l1;
 }
 
+template <bool TENSOR>
 __global__ __launch_bounds__(256) void hv_lap2_pre_kernel(HvArgs a) {
   int ie, k, pt;
   if (!slab_of(gridDim.x, a.nelem, a.nlev, ie, k, pt)) return;
@@ -49,11 +50,20 @@ __global__ __launch_bounds__(256) void hv_lap2_pre_kernel(HvArgs a) {
   load_dvv(a.m.dvv, pt, d);
   const size_t s = ((size_t)ie * a.nlev + k) * 16 + pt;
   const size_t v = ((size_t)ie * a.nlev + k) * 32 + pt;
-  // second laplacian (consthv)
-  double tt = sph_laplace(e, d, a.tt_b[s]);
-  double dpt = sph_laplace(e, d, a.dpt_b[s]);
-  double vt0, vt1;
-  sph_vlaplace_contra(e, d, a.nu_ratio2, a.vt_b[v], a.vt_b[v + 16], vt0, vt1);
+  // second laplacian: constant coefficient, or the tensor form
+  // (HyperviscosityFunctorImpl.hpp:88-128)
+  double tt, dpt, vt0, vt1;
+  if (TENSOR) {
+    double tv[4], vs[6];
+    load_tensor(a.m, ie, pt, tv, vs);
+    tt = sph_laplace_tensor(e, d, tv, a.tt_b[s]);
+    dpt = sph_laplace_tensor(e, d, tv, a.dpt_b[s]);
+    sph_vlaplace_cartesian(e, d, tv, vs, a.vt_b[v], a.vt_b[v + 16], vt0, vt1);
+  } else {
+    tt = sph_laplace(e, d, a.tt_b[s]);
+    dpt = sph_laplace(e, d, a.dpt_b[s]);
+    sph_vlaplace_contra(e, d, a.nu_ratio2, a.vt_b[v], a.vt_b[v + 16], vt0, vt1);
+  }
   // pre-exchange
   const double dpn = a.dp[s];
   a.dpdiss_ave[s] += a.eta_ave_w * dpn / a.nsub;
@@ -157,23 +167,36 @@ __global__ __launch_bounds__(256) void hv_lap2_pre_mixed_kernel(HvArgs a) {
   a.dpt_a[s] = dpt;
 }
 
-__global__ __launch_bounds__(256) void hv_update_kernel(HvArgs a) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)a.nelem * a.nlev * 16) return;
-  const size_t row = i >> 4;
-  const int pt = i & 15;
-  const int ie = (int)(row / a.nlev);
+// Second exchange fused with the update: the assembled tensors are used at
+// once by the lane that assembled them (same gather order as dss_kernel).
+__global__ __launch_bounds__(256) void hv_dss_update_kernel(HvArgs a) {
+  int ie, k, pt;
+  if (!group_slab(a.nelem, a.nlev, ie, k, pt)) return;
+  const int *src = a.src + ((size_t)ie * 16 + pt) * 3;
+  const int s0 = src[0], s1 = src[1], s2 = src[2];
+  const size_t es = (size_t)a.nlev * 16, ev = (size_t)a.nlev * 32;
+  auto g = [&](const double *f, size_t estr, size_t lstr, size_t c) {
+    const size_t off = (size_t)k * lstr + c;
+    double y = f[(size_t)ie * estr + off + pt];
+    if (s0 >= 0) y += f[(size_t)(s0 >> 4) * estr + off + (s0 & 15)];
+    if (s1 >= 0) y += f[(size_t)(s1 >> 4) * estr + off + (s1 & 15)];
+    if (s2 >= 0) y += f[(size_t)(s2 >> 4) * estr + off + (s2 & 15)];
+    return y;
+  };
+  const double b0 = g(a.vt_a, ev, 32, 0), b1 = g(a.vt_a, ev, 32, 16);
+  const double bt = g(a.tt_a, es, 16, 0), bdp = g(a.dpt_a, es, 16, 0);
   const double r = a.m.rspheremp[(size_t)ie * 16 + pt];
-  const size_t v0 = row * 32 + pt, v1 = v0 + 16;
-  const double vt0 = (a.dt * a.vt_b[v0] * r);
-  const double vt1 = (a.dt * a.vt_b[v1] * r);
+  const size_t i = ((size_t)ie * a.nlev + k) * 16 + pt;
+  const size_t v0 = ((size_t)ie * a.nlev + k) * 32 + pt, v1 = v0 + 16;
+  const double vt0 = (a.dt * b0 * r);
+  const double vt1 = (a.dt * b1 * r);
   const double u = a.v[v0] + vt0, w = a.v[v1] + vt1;
   a.v[v0] = u;
   a.v[v1] = w;
-  const double tt = (a.dt * a.tt_b[i] * r);
+  const double tt = (a.dt * bt * r);
   const double heating = vt0 * u + vt1 * w;
   a.t[i] = a.t[i] + tt - heating / CP;
-  a.dp[i] = (a.dpt_b[i] * r);
+  a.dp[i] = (bdp * r);
 }
 
 void launch_hv_cycle(hipStream_t st, const HvArgs &a) {
@@ -200,13 +223,9 @@ void launch_hv_cycle(hipStream_t st, const HvArgs &a) {
   }
   launch_dss(st, x);
   if (a.mixed) hipLaunchKernelGGL(hv_lap2_pre_mixed_kernel, dim3(blocks), dim3(256), 0, st, a);
-  else hipLaunchKernelGGL(hv_lap2_pre_kernel, dim3(blocks), dim3(256), 0, st, a);
-  x.rsmp = nullptr;
-  x.f32 = 0;
-  x.f[1] = {a.vt_a + 16, a.vt_b + 16, ev, 32};
-  launch_dss(st, x);
-  const size_t n = nslab * 16;
-  hipLaunchKernelGGL(hv_update_kernel, dim3((n + 255) / 256), dim3(256), 0, st, a);
+  else if (a.consthv) hipLaunchKernelGGL(hv_lap2_pre_kernel<false>, dim3(blocks), dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(hv_lap2_pre_kernel<true>, dim3(blocks), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(hv_dss_update_kernel, dim3(group_blocks_x(a.nelem, a.nlev)), dim3(256), 0, st, a);
 }
 
 }  // namespace hx

@@ -18,6 +18,9 @@ struct CaarArgs {
   const double *qdp_n0;  // moist: tracer block of the n0_qdp level (tracer 0 used), else nullptr
   double *vn0, *omega, *phi;
   double *divv;                 // scratch: div(v dp) per level
+  int rsplit0;                  // rsplit == 0: Eulerian vertical advection
+  const double *hybi;           // nlev+1 interface b coefficients (rsplit == 0)
+  double *eta_acc;              // eta_dot_dpdn accumulator (rsplit == 0)
   double *out_v, *out_t, *out_dp;
   double dt, eta_ave_w, p_top;  // p_top = hyai(0)*ps0
 };
@@ -48,6 +51,7 @@ struct HvArgs {
   double dt, eta_ave_w, nu, nu_s, nu_p, nu_top, nu_ratio1, nu_ratio2;
   int nsub;
   int mixed;                        // FP32 laplacians, FP32 first-laplacian buffers
+  int consthv;                      // 0: tensor form of the second laplacian
 };
 
 struct EulerArgs {
@@ -65,11 +69,18 @@ struct EulerArgs {
   double dt, rhs_multiplier, rhs_viss, nu_p, nu_q;
   int limiter_option;
   const double *qdp_avg_n0;   // stage 3: Qdp(n0_qdp) for the time average, else nullptr
-  int mixed;                  // FP32 flux divergence and biharmonic (qbih, qbih_pre hold floats)
+  const double *qdp_pre_in, *fdss_pre_in;  // stage-1 pre-exchange buffers gathered by the fused stage 2
+  int mixed;                  // FP32 flux divergence
+  int bih32;                  // FP32 tracer biharmonic (qbih, qbih_pre hold floats)
+  int consthv;                // 0: tensor laplacian in the biharmonic's second half
 };
 
 struct RemapArgs {
   int nelem, nlev, qsize, remap_alg;
+  int nstates;                // 3: remap v, T and the tracers; 0 (rsplit == 0): tracers only
+  const double *eta;          // rsplit == 0: eta_dot_dpdn, source = target + dt*(eta(k+1) - eta(k))
+  double *src;                // rsplit == 0: scratch for the source thickness
+  double dt;
   double *v, *t;              // state at np1
   const double *dp;           // source thickness dp3d(np1)
   double *ps;                 // ps_v(np1) [ie][pt]
@@ -80,7 +91,46 @@ struct RemapArgs {
   int *error;                 // set to 1 on invalid thickness, 2 on unsupported deformation
 };
 
+// Adjoint of one CAAR stage (dry, rsplit = 0).  The stage is
+//   x_out = DSS_r( sph*(x_a + h*RHS(x_b)) );
+// given lp = DSS_r(adjoint of x_out), the kernels add the adjoint of x_b into
+// b (overwrite or accumulate) and c1*sph*lp into a1 (and c2*sph*lp into a2).
+struct AdjCaarArgs {
+  MetricPtrs m;
+  int nelem, nlev;
+  const double *v, *t, *dp;            // the stage's x_b
+  const double *lp_v, *lp_t, *lp_dp;   // DSS_r of the output adjoint
+  const double *hybi;                  // nlev+1
+  double h, p_top;
+  // scratch (one 3-D scalar each; stot is 2-D)
+  double *div, *p, *c, *stot, *pb, *phib, *etab, *ombp, *divb, *lv, *lt, *ldp;
+  double *b_v, *b_t, *b_dp;
+  int b_acc;
+  double *a1_v, *a1_t, *a1_dp;
+  double c1;
+  double *a2_v, *a2_t, *a2_dp;         // optional (a2_v == nullptr: none)
+  double c2;
+};
+
+// Adjoint of one hyperviscosity cycle: x -> xo, adjoint l (in/out, in place)
+struct AdjHvArgs {
+  MetricPtrs m;
+  int nelem, nlev;
+  const double *x_v, *xo_v;            // velocity before and after the cycle
+  double *l_v, *l_t, *l_dp;
+  double *s_v, *s_t, *s_dp;            // DSS inputs
+  const double *m_v, *m_t, *m_dp;      // DSS outputs
+  double dt, nu, nu_s, nu_p, nu_top, nu_ratio1, nu_ratio2;
+};
+
 // launchers (each enqueues on `st`)
+void launch_adj_caar(hipStream_t st, const AdjCaarArgs &a);
+void launch_adj_hv_pre(hipStream_t st, const AdjHvArgs &a);
+void launch_adj_hv_a(hipStream_t st, const AdjHvArgs &a);
+void launch_adj_hv_b(hipStream_t st, const AdjHvArgs &a);
+void launch_adj_fill_dp(hipStream_t st, int nelem, int nlev, const double *lps, double *ldp);
+void launch_adj_ps(hipStream_t st, int nelem, int nlev, const double *hybi, const double *ldp, double *lps);
+void launch_adj_zero(hipStream_t st, size_t n, double *p);
 void launch_dp3d_from_ps(hipStream_t st, int nelem, int nlev, const double *hyai_d, const double *hybi_d, double ps0,
                          const double *ps, double *dp);
 void launch_prim_step_init(hipStream_t st, int nelem, int nlev, bool zero_diss, const double *dp_n0,
@@ -93,6 +143,7 @@ void launch_hv_cycle(hipStream_t st, const HvArgs &a);
 void launch_euler_divdp_qmm(hipStream_t st, const EulerArgs &a);
 void launch_euler_nbr_minmax(hipStream_t st, const EulerArgs &a);
 void launch_euler_advect(hipStream_t st, const EulerArgs &a, int stage);
+void launch_euler_advect_gather(hipStream_t st, const EulerArgs &a);
 void launch_euler_dss(hipStream_t st, const EulerArgs &a, int avg);
 void launch_euler_qmm_lap(hipStream_t st, const EulerArgs &a);
 void launch_euler_bih_post(hipStream_t st, const EulerArgs &a);

@@ -15,7 +15,9 @@
 
 namespace hx {
 
-// ps_v = sum(dp3d) + hyai0*ps0 and the thickness check (ComputeThicknessTag)
+// ps_v = sum(dp3d) + hyai0*ps0 and the thickness check (ComputeThicknessTag).
+// rsplit == 0: the source thickness of the tracer remap is the target
+// thickness plus dt*(eta_dot_dpdn(k+1) - eta_dot_dpdn(k)) (RemapFunctor.hpp:62-90).
 __global__ __launch_bounds__(256) void remap_thickness_kernel(RemapArgs a) {
   const size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= (size_t)a.nelem * 16) return;
@@ -26,10 +28,22 @@ __global__ __launch_bounds__(256) void remap_thickness_kernel(RemapArgs a) {
   for (int k = 0; k < a.nlev; ++k) {
     const double x = dp[(size_t)k * 16];
     s += x;
-    bad |= isnan(x) || x < 0.0;
+    if (a.nstates) bad |= isnan(x) || x < 0.0;
   }
   s += a.hyai0 * a.ps0;
   a.ps[col] = s;
+  if (!a.nstates) {
+    const double *eta = a.eta + (size_t)ie * a.nlev * 16 + pt;
+    double *src = a.src + (size_t)ie * a.nlev * 16 + pt;
+    for (int k = 0; k < a.nlev; ++k) {
+      const double tgt = (a.hyai[k + 1] - a.hyai[k]) * a.ps0 + (a.hybi[k + 1] - a.hybi[k]) * s;
+      const double en = k + 1 < a.nlev ? eta[(size_t)(k + 1) * 16] : 0;
+      const double delta = en - eta[(size_t)k * 16];
+      const double x = tgt + a.dt * delta;
+      src[(size_t)k * 16] = x;
+      bad |= isnan(x) || x < 0.0;
+    }
+  }
   if (bad) atomicOr(a.error, 1);
 }
 
@@ -95,17 +109,18 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a, int nfb) {
   const int ie = eg * 4 + ((threadIdx.x >> 4) & 3);
   if (ie >= a.nelem) return;
   const int pt = threadIdx.x & 15;
-  const int nf = 3 + a.qsize;
+  const int nf = 3 + a.qsize;  // in the field numbering below (0,1: v, 2: T, 3..: tracers)
+  const int f0 = 3 - a.nstates;  // rsplit == 0: tracers only
   const size_t col = (size_t)ie * 16 + pt;
   const int L = a.nlev;
-  const double *src = a.dp + (size_t)ie * L * 16 + pt;
+  const double *src = (a.nstates ? a.dp : a.src) + (size_t)ie * L * 16 + pt;
   double *var[RFB];
   double *Qout[RFB];
   size_t vs[RFB];
   bool on[RFB], state[RFB];
 #pragma unroll
   for (int i = 0; i < RFB; ++i) {
-    const int f = fb * RFB + i;
+    const int f = f0 + fb * RFB + i;
     on[i] = f < nf;
     state[i] = f < 3;
     vs[i] = f < 2 ? 32 : 16;
@@ -216,8 +231,8 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a, int nfb) {
 
   // ------------------------------------------------------------ sweep
   // cell j+3 enters the window at iteration j; its dx and field values are
-  // loaded one iteration ahead so the load latency hides behind a sweep step
-  double px = 0.0, pv[RFB];
+  // loaded two iterations ahead so the load latency hides behind sweep steps
+  double px = 0.0, pv[RFB], qx = 0.0, qv[RFB];
   auto fetch = [&](int m, double &x, double *v) {
     if (m <= L) {
       x = src[(size_t)(m - 1) * 16];
@@ -227,9 +242,10 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a, int nfb) {
     }
   };
   fetch(3, px, pv);
+  fetch(4, qx, qv);
   for (int j = 0; j <= L; ++j) {
     double nx = 0.0, nv[RFB];
-    fetch(j + 4, nx, nv);
+    fetch(j + 5, nx, nv);
     // dma(j+1) factors: cells j..j+2; ai(j) factors: cells j-1..j+2
     const double h0 = X1 / (X0 + X1 + X2);
     const double h1 = (2.0 * X0 + X1) / (X1 + X2);
@@ -300,10 +316,12 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a, int nfb) {
       V0[i] = V1[i]; V1[i] = V2[i]; V2[i] = vn;
       ai_prev[i] = ai_j[i];
       dma_j[i] = dma_j1[i];
-      pv[i] = nv[i];
+      pv[i] = qv[i];
+      qv[i] = nv[i];
     }
     Xm1 = X0; X0 = X1; X1 = X2; X2 = xn;
-    px = nx;
+    px = qx;
+    qx = nx;
   }
   if (k < L) atomicOr(a.error, 2);
 }
@@ -319,7 +337,8 @@ void launch_remap(hipStream_t st, const RemapArgs &a) {
   // four fields per lane: the column work (grid factors, target search,
   // control flow) is shared four ways at two waves per SIMD
   constexpr int fb = 4;
-  const int nfb = (3 + a.qsize + fb - 1) / fb;
+  const int nfb = (a.nstates + a.qsize + fb - 1) / fb;
+  if (nfb == 0) return;
   const int nwaves = ((a.nelem + 3) / 4) * nfb;
   const int nb = (((nwaves + 3) / 4) + 7) & ~7;
   hipLaunchKernelGGL(remap_kernel<fb>, dim3(nb), dim3(256), 0, st, a, nfb);

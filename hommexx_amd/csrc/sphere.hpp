@@ -21,6 +21,7 @@ struct MetricPtrs {
   const double *D, *Dinv, *metinv;           // [ie][4][16]
   const double *fcor, *mp, *spheremp, *rspheremp, *metdet, *phis;  // [ie][16]
   const double *dvv;                          // [4][4], dvv[a][b] = L_b'(x_a)
+  const double *tensorvisc, *vec_sph2cart;    // [ie][4][16], [ie][6][16] (tensor hyperviscosity)
 };
 
 __device__ __forceinline__ void load_elem(const MetricPtrs &m, int ie, int pt, ElemLane &e) {
@@ -166,6 +167,35 @@ __device__ __forceinline__ void sph_vlaplace_contra(const ElemLane &e, const Dvv
   const double f = 2.0 * e.sph;
   l0 = f * u0 * re2 + gc0;
   l1 = f * u1 * re2 + gc1;
+}
+
+// tensor hyperviscosity: T(idim,jdim) = tv[idim*2+jdim], vec_sph2cart(c,i) = vs[c*3+i]
+__device__ __forceinline__ void load_tensor(const MetricPtrs &m, int ie, int pt, double tv[4], double vs[6]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) tv[k] = m.tensorvisc[(size_t)ie * 64 + 16 * k + pt];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) vs[k] = m.vec_sph2cart[(size_t)ie * 96 + 16 * k + pt];
+}
+
+// laplace_tensor (SphereOperators.hpp:602-635)
+__device__ __forceinline__ double sph_laplace_tensor(const ElemLane &e, const DvvLane &d, const double tv[4], double f) {
+  double g0, g1;
+  sph_grad(e, d, f, g0, g1);
+  const double s0 = tv[0] * g0 + tv[2] * g1;
+  const double s1 = tv[1] * g0 + tv[3] * g1;
+  return sph_div_wk(e, d, s0, s1);
+}
+
+// vlaplace_sphere_wk_cartesian (:750-814): tensor laplacians of the three
+// cartesian components, projected back, plus the undamping 2 spheremp v / a^2
+__device__ __forceinline__ void sph_vlaplace_cartesian(const ElemLane &e, const DvvLane &d, const double tv[4],
+                                                       const double vs[6], double v0, double v1, double &l0,
+                                                       double &l1) {
+  const double a = sph_laplace_tensor(e, d, tv, vs[0] * v0 + vs[3] * v1);
+  const double b = sph_laplace_tensor(e, d, tv, vs[1] * v0 + vs[4] * v1);
+  const double c = sph_laplace_tensor(e, d, tv, vs[2] * v0 + vs[5] * v1);
+  l0 = vs[0] * a + vs[1] * b + vs[2] * c + 2.0 * e.sph * v0 * (RREARTH) * (RREARTH);
+  l1 = vs[3] * a + vs[4] * b + vs[5] * c + 2.0 * e.sph * v1 * (RREARTH) * (RREARTH);
 }
 
 }  // namespace hx

@@ -34,6 +34,13 @@ def load():
     """Load the shared library (built by ``__graft_entry__.build()``)."""
     global _lib
     if _lib is None:
+        # torch ships its own libamdhip64.so.7: load it first, so the library
+        # binds to the same HIP runtime (loaded after the library, torch would
+        # start a second runtime that does not see the GPU)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise HommexxError("libhommexx_amd.so is not built: run __graft_entry__.build() (hipcc, gfx950)")
         L = ctypes.CDLL(LIB_PATH)

@@ -132,6 +132,21 @@ int hommexx_hip_profile_get(const char *name, int *count, double *total_ms);
 int hommexx_hip_profile_names(char *buf, int len);
 void hommexx_hip_time_levels(int *nm1, int *n0, int *np1, int *nstep, int *n0_qdp, int *np1_qdp);
 
+/* Differentiable form (the reference has no adjoint; these serve the
+ * torch.autograd harness hommexx_amd/adjoint.py).  Device pointers in the
+ * library's layouts: v [ie][lev][2][pt], T [ie][lev][pt], ps [ie][pt]
+ * (pt = igp*4 + jgp).  get/set copy the prognostic state of time level n0
+ * (set also recomputes dp3d(n0) from ps).  hommexx_hip_adjoint_step takes
+ * the input state of one prim_run_subcycle_c step of length dt and, in
+ * place, maps the adjoint of the step's output (v, T, ps) to the adjoint of
+ * its input; it needs a dry rsplit = 0, qsplit = 1 configuration with
+ * constant-coefficient hyperviscosity, no forcing and the FP64 form, and
+ * leaves the session's model state unchanged.  Return 0 on success. */
+int hommexx_hip_get_state(double *v, double *T, double *ps);
+int hommexx_hip_set_state(const double *v, const double *T, const double *ps);
+int hommexx_hip_adjoint_step(double dt, const double *v, const double *T, const double *ps, double *lam_v,
+                             double *lam_T, double *lam_ps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,7 @@ def test_ten_step_parity():
     dict(limiter_option=9),
     dict(remap_alg=2),
     dict(rsplit=3),
+    dict(rsplit=0),
     dict(nu_div=2.5e16),
     dict(moisture=True),
     dict(qsplit=2, hypervis_subcycle=2),
@@ -96,6 +97,24 @@ def test_forcing_parity(ftype, moist):
     print(ftype, moist, fmt(rep))
     for k, (e, _) in rep.items():
         assert e <= TOL1 * 10, (ftype, moist, k, e)
+
+
+def test_tensor_hyperviscosity_parity():
+    """hypervis_scaling != 0: tensor second laplacian (HV and tracer
+    biharmonic) with a seeded symmetric positive tensor per point."""
+    prob = build_problem(ne=4, nlev=26, qsize=3, seed=8, hypervis_scaling=3.2)
+    rng = np.random.default_rng(8)
+    a = rng.uniform(0.5, 1.5, (prob.nelem, 4, 4))
+    c = rng.uniform(0.5, 1.5, (prob.nelem, 4, 4))
+    b = rng.uniform(-0.3, 0.3, (prob.nelem, 4, 4))
+    tv = np.zeros((prob.nelem, 2, 2, 4, 4))
+    tv[:, 0, 0], tv[:, 1, 1], tv[:, 0, 1], tv[:, 1, 0] = a, c, b, b
+    prob.metric["tensorvisc"] = tv
+    out, ref = run_both(prob, 2)
+    rep = compare(out, ref)
+    print(fmt(rep))
+    for k, (e, _) in rep.items():
+        assert e <= TOL1 * 10, (k, e)
 
 
 def test_l72_parity():

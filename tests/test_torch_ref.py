@@ -1,0 +1,28 @@
+"""The pure-PyTorch FP64 step (the adjoint's reference gradient) against the
+CPU oracle: same dry rsplit = 0 step on the same seeded inputs."""
+import pytest
+import torch
+
+from hommexx_amd.problem import build_problem
+from oracle import OracleSession
+from parity import rel_l2
+from torch_ref import TorchStep, initial_state
+
+
+@pytest.mark.parametrize("nu_top,nsteps", [(0.0, 1), (2.5e5, 3)])
+def test_torch_step_matches_oracle(nu_top, nsteps):
+    prob = build_problem(ne=4, nlev=26, qsize=0, seed=3, rsplit=0, nu_top=nu_top, t_noise=0.1)
+    o = OracleSession(prob)
+    _, _, n0, _ = o.run(nsteps)
+    ref = o.state()
+    o.close()
+    with torch.no_grad():
+        v, T, ps = TorchStep(prob).run(*initial_state(prob), nsteps)
+    for name, a, b in (("v", v, ref["v"][:, n0]), ("T", T, ref["T"][:, n0]), ("ps_v", ps, ref["ps_v"][:, n0])):
+        e = rel_l2(a.numpy(), b)
+        assert e <= 1e-13, (name, e)
+
+
+def test_torch_step_rejects_other_configurations():
+    with pytest.raises(ValueError):
+        TorchStep(build_problem(ne=2, nlev=26, qsize=0, rsplit=1))

@@ -1,0 +1,245 @@
+"""Pure-PyTorch FP64 restatement of the dry preqx dynamics step with
+rsplit = 0 (Eulerian vertical coordinate), used as the reference gradient
+for the HIP adjoint through ``torch.autograd``.
+
+TEST INFRASTRUCTURE ONLY.  It restates, in vectorised torch, the same step
+the CPU oracle runs (``oracle/preqx_oracle.c``) for a configuration without
+moisture: dp3d from ps (prim_driver.cpp:101-110), the five CAAR stages of
+u3_5stage with their DSS (prim_advance_exp.cpp:113-161,
+CaarFunctorImpl.hpp:98-729), the subcycled biharmonic hyperviscosity with
+the nu_top sponge (HyperviscosityFunctorImpl.cpp, biharmonic_wk_dp3d) and
+the rsplit = 0 surface-pressure update of the remap
+(PpmRemap.hpp / VerticalRemapManager.cpp: ps_v = hyai(1) ps0 + sum dp3d).
+Tracers do not feed back on the dynamics of a dry run, so they are left out.
+
+Fields use the device layout of the library: scalars ``(nelem, nlev, 4, 4)``
+indexed [ie][lev][igp][jgp], vectors ``(nelem, nlev, 2, 4, 4)``, surface
+pressure ``(nelem, 4, 4)``.
+"""
+import numpy as np
+import torch
+
+RGAS = 287.04
+CP = 1005.0
+KAPPA = RGAS / CP
+RRE = 1.0 / 6.376e6
+
+
+class TorchStep:
+    """One dry rsplit=0 step as a differentiable torch function."""
+
+    def __init__(self, prob, device="cpu"):
+        p = prob.params
+        if p.rsplit != 0 or p.moisture or p.hypervis_scaling != 0.0 or p.qsplit != 1:
+            raise ValueError("the torch reference covers dry rsplit=0, qsplit=1, constant-coefficient HV")
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=device)
+        m = prob.metric
+        self.L = prob.nlev
+        self.dt = prob.dt
+        self.D = t(m["D"]).unsqueeze(3)  # (E, 2, 2, 1, 4, 4): D(a, b)
+        self.Dinv = t(m["Dinv"]).unsqueeze(3)
+        self.metinv = t(m["metinv"]).unsqueeze(3)
+        e2 = lambda a: t(a).unsqueeze(1)  # (E, 1, 4, 4)
+        self.metdet = e2(m["metdet"])
+        self.sph = e2(m["spheremp"])
+        self.rsph = e2(m["rspheremp"])
+        self.mp = e2(m["mp"])
+        self.fcor = e2(m["fcor"])
+        self.phis = e2(prob.state["phis"])
+        self.dvv = t(prob.dvv)  # dvv[a][b]
+        vc = prob.vcoord
+        self.ps0 = vc.ps0
+        self.ptop = vc.hyai[0] * vc.ps0
+        self.dA = t((vc.hyai[1:] - vc.hyai[:-1]) * vc.ps0).view(1, -1, 1, 1)
+        self.dB = t(vc.hybi[1:] - vc.hybi[:-1]).view(1, -1, 1, 1)
+        self.hybi = t(vc.hybi[:-1]).view(1, -1, 1, 1)  # hybi(k) at interface k = 0..L-1
+        self.nu, self.nu_s, self.nu_p, self.nu_top = p.nu, p.nu_s, p.nu_p, p.nu_top
+        self.nsub = p.hypervis_subcycle
+        if p.nu != p.nu_div:
+            self.nr1 = self.nr2 = p.nu_div / p.nu
+        else:
+            self.nr1 = self.nr2 = 1.0
+        # DSS: every GLL copy of a geometric node sums into one slot
+        key = np.round(prob.mesh.pos * 1e9).astype(np.int64).reshape(-1, 3)
+        _, inv = np.unique(key, axis=0, return_inverse=True)
+        self.node = torch.as_tensor(inv.reshape(-1), device=device)
+        self.nnode = int(inv.max()) + 1
+
+    # ------------------------------------------------------ element operators
+    def _rowc(self, s):  # sum_k dvv[jg, k] s[ig, k]
+        return torch.einsum("...ik,jk->...ij", s, self.dvv)
+
+    def _colc(self, s):  # sum_k dvv[ig, k] s[k, jg]
+        return torch.einsum("...kj,ik->...ij", s, self.dvv)
+
+    def grad(self, s):
+        Di = self.Dinv
+        v0 = self._rowc(s) * RRE
+        v1 = self._colc(s) * RRE
+        return Di[:, 0, 0] * v0 + Di[:, 0, 1] * v1, Di[:, 1, 0] * v0 + Di[:, 1, 1] * v1
+
+    def div(self, a, b):
+        Di, md = self.Dinv, self.metdet
+        gv0 = (Di[:, 0, 0] * a + Di[:, 1, 0] * b) * md
+        gv1 = (Di[:, 0, 1] * a + Di[:, 1, 1] * b) * md
+        return (self._rowc(gv0) + self._colc(gv1)) * (1.0 / md * RRE)
+
+    def vort(self, u, v):
+        D, md = self.D, self.metdet
+        c0 = D[:, 0, 0] * u + D[:, 0, 1] * v
+        c1 = D[:, 1, 0] * u + D[:, 1, 1] * v
+        return (self._rowc(c1) - self._colc(c0)) * (1.0 / md * RRE)
+
+    def _wk(self, x):
+        """(A, B) with A[n,m] = sum_j x[n,j] dvv[j,m], B[n,m] = sum_j x[j,m] dvv[j,n]"""
+        return torch.einsum("...nj,jm->...nm", x, self.dvv), torch.einsum("...jm,jn->...nm", x, self.dvv)
+
+    def div_wk(self, v0, v1):
+        Di = self.Dinv
+        a = Di[:, 0, 0] * v0 + Di[:, 1, 0] * v1
+        b = Di[:, 0, 1] * v0 + Di[:, 1, 1] * v1
+        A, _ = self._wk(self.sph * a)
+        _, B = self._wk(self.sph * b)
+        return -(A + B) * RRE
+
+    def lap(self, f):
+        return self.div_wk(*self.grad(f))
+
+    def grad_wk_testcov(self, s):
+        mi, md, D = self.metinv, self.metdet, self.D
+        A, B = self._wk(self.mp * s)
+        b0 = -md * (mi[:, 0, 0] * A + mi[:, 0, 1] * B)
+        b1 = -md * (mi[:, 1, 0] * A + mi[:, 1, 1] * B)
+        return (D[:, 0, 0] * b0 + D[:, 1, 0] * b1) * RRE, (D[:, 0, 1] * b0 + D[:, 1, 1] * b1) * RRE
+
+    def curl_wk_testcov(self, s):
+        D = self.D
+        A, B = self._wk(self.mp * s)
+        sb0, sb1 = -B, A
+        return (D[:, 0, 0] * sb0 + D[:, 1, 0] * sb1) * RRE, (D[:, 0, 1] * sb0 + D[:, 1, 1] * sb1) * RRE
+
+    def vlap(self, u0, u1, nr):
+        d = self.div(u0, u1)
+        if nr > 0 and nr != 1.0:
+            d = d * nr
+        g0, g1 = self.grad_wk_testcov(d)
+        c0, c1 = self.curl_wk_testcov(self.vort(u0, u1))
+        f = 2.0 * self.sph * RRE * RRE
+        return f * u0 + (g0 - c0), f * u1 + (g1 - c1)
+
+    # ------------------------------------------------------------------ DSS
+    def assemble(self, x):
+        """sum of all copies of every node, x: (E, nl, 4, 4)"""
+        E, nl = x.shape[0], x.shape[1]
+        flat = x.permute(0, 2, 3, 1).reshape(E * 16, nl)
+        s = torch.zeros(self.nnode, nl, dtype=x.dtype, device=x.device).index_add(0, self.node, flat)
+        return s[self.node].reshape(E, 4, 4, nl).permute(0, 3, 1, 2)
+
+    def dss_r(self, x):
+        return self.rsph * self.assemble(x)
+
+    # ----------------------------------------------------------------- CAAR
+    def caar(self, am, bn, h):
+        """pre-DSS outputs sph*(x(nm1) + h*RHS(x(n0))), rsplit = 0, dry"""
+        vm, Tm, dpm = am
+        v, T, dp = bn
+        u0, u1 = v[:, :, 0], v[:, :, 1]
+        L = self.L
+        divv = self.div(u0 * dp, u1 * dp)
+        excl = lambda x: torch.cumsum(x, 1) - x  # sum_{j<k}
+        p = self.ptop + excl(dp) + 0.5 * dp
+        r = RGAS * T * (dp * 0.5 / p)
+        suffix = torch.flip(torch.cumsum(torch.flip(r, [1]), 1), [1]) - r  # sum_{j>k}
+        phi = self.phis + 2.0 * suffix + r
+        gp0, gp1 = self.grad(p)
+        omb = (u0 * gp0 + u1 * gp1 - (excl(divv) + 0.5 * divv)) / p
+        stot = divv.sum(1, keepdim=True)
+        eta = self.hybi * stot - excl(divv)
+        eta = torch.cat([torch.zeros_like(eta[:, :1]), eta[:, 1:]], 1)  # eta(0) = 0
+        eta_up = torch.cat([eta[:, 1:], torch.zeros_like(eta[:, :1])], 1)  # eta(k+1), 0 at the bottom
+        facp = 0.5 * (1 / dp) * eta_up
+        facm = 0.5 * (1 / dp) * eta
+
+        def vadv(f):
+            fp = torch.cat([f[:, 1:], f[:, -1:]], 1)
+            fm = torch.cat([f[:, :1], f[:, :-1]], 1)
+            return facp * (fp - f) + facm * (f - fm)
+
+        gT0, gT1 = self.grad(T)
+        ttens = -vadv(T) - (u0 * gT0 + u1 * gT1) + KAPPA * T * omb
+        Tout = self.sph * (Tm + h * ttens)
+        coef = RGAS * (T / p)
+        e0, e1 = self.grad(0.5 * (u0 * u0 + u1 * u1) + phi)
+        e0 = coef * gp0 + e0
+        e1 = coef * gp1 + e1
+        vort = self.vort(u0, u1) + self.fcor
+        du0 = -e0 - vadv(u0) + u1 * vort
+        du1 = -e1 - vadv(u1) - u0 * vort
+        vout = self.sph.unsqueeze(2) * (vm + h * torch.stack([du0, du1], 2))
+        dpout = self.sph * (dpm - h * (eta_up + divv - eta))
+        return vout, Tout, dpout
+
+    def stage(self, am, bn, h):
+        v, T, dp = self.caar(am, bn, h)
+        v = torch.stack([self.dss_r(v[:, :, 0]), self.dss_r(v[:, :, 1])], 2)
+        return v, self.dss_r(T), self.dss_r(dp)
+
+    # -------------------------------------------------------- hyperviscosity
+    def hv_cycle(self, x, dt):
+        v, T, dp = x
+        u0, u1 = v[:, :, 0], v[:, :, 1]
+        l0, l1 = self.vlap(u0, u1, self.nr1)
+        lt, ld = self.lap(T), self.lap(dp)
+        l0, l1, lt, ld = (self.dss_r(a) for a in (l0, l1, lt, ld))
+        l0, l1 = self.vlap(l0, l1, self.nr2)
+        lt, ld = self.lap(lt), self.lap(ld)
+        v0 = -self.nu * l0
+        v1 = -self.nu * l1
+        tt = -self.nu_s * lt
+        dpt = -self.nu_p * ld
+        if self.nu_top > 0:
+            nt = min(3, self.L)
+            sc = torch.tensor([4.0, 2.0, 1.0][:nt], dtype=torch.float64, device=T.device).view(1, nt, 1, 1) \
+                * self.nu_top
+            t0, t1 = self.vlap(u0[:, :nt], u1[:, :nt], 1.0)
+            top = lambda a, b: torch.cat([a[:, :nt] + sc * b, a[:, nt:]], 1)
+            v0, v1 = top(v0, t0), top(v1, t1)
+            tt = top(tt, self.lap(T[:, :nt]))
+            dpt = top(dpt, self.lap(dp[:, :nt]))
+        dpt = dpt * dt + dp * self.sph
+        v0, v1, tt, dpt = (self.assemble(a) for a in (v0, v1, tt, dpt))
+        r = self.rsph
+        vt0, vt1 = dt * v0 * r, dt * v1 * r
+        n0, n1 = u0 + vt0, u1 + vt1
+        heating = vt0 * n0 + vt1 * n1
+        Tn = T + dt * tt * r - heating / CP
+        return torch.stack([n0, n1], 2), Tn, dpt * r
+
+    # ----------------------------------------------------------------- step
+    def step(self, v, T, ps):
+        dt = self.dt
+        dp = self.dA + self.dB * ps.unsqueeze(1)
+        u0 = (v, T, dp)
+        u1 = self.stage(u0, u0, dt / 5.0)
+        u2 = self.stage(u0, u1, dt / 5.0)
+        u3 = self.stage(u0, u2, dt / 3.0)
+        u4 = self.stage(u0, u3, 2.0 * dt / 3.0)
+        w = tuple((5.0 * a - b) / 4.0 for a, b in zip(u1, u0))
+        x = self.stage(w, u4, 3.0 * dt / 4.0)
+        for _ in range(self.nsub):
+            x = self.hv_cycle(x, dt / self.nsub)
+        vn, Tn, dpn = x
+        return vn, Tn, self.ptop + dpn.sum(1)
+
+    def run(self, v, T, ps, nsteps):
+        for _ in range(nsteps):
+            v, T, ps = self.step(v, T, ps)
+        return v, T, ps
+
+
+def initial_state(prob, device="cpu"):
+    """(v, T, ps) of time level n0 of ``prob`` in the device layout."""
+    s = prob.state
+    n0 = prob.time_level[1] - 1
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=device)
+    return t(s["v"][:, n0]), t(s["T"][:, n0]), t(s["ps_v"][:, n0])
