@@ -51,16 +51,19 @@ def kernel_bytes(name, nelem, nlev, q, launches):
     F = f3_bytes(nelem, nlev)
     ql = q / 8.0
     if name == "euler_advect":
-        # reads Qdp(Q), derived_dp, divdp_proj, vn0(2), divdp, fdss, qlim; writes qdp_pre(Q), fdss_pre
-        # stage 2 also writes qlim; stage 3 also reads qbih(Q) and dpdiss_bih
+        # reads Qdp(Q), derived_dp, divdp_proj, vn0(2), divdp, fdss, qlim; writes qdp_pre(Q), fdss_pre.
+        # Stage 2 also writes qlim; it reads stage 1's pre-exchange buffers (Q + 1) in place of the
+        # assembled Qdp and divdp_proj and writes the assembled divdp_proj (its exchange is fused in).
+        # Stage 3 also reads qbih(Q) and dpdiss_bih.
         s1 = (q + 6 + ql) + (q + 1)
-        s2 = s1 + ql
+        s2 = s1 + ql + 2
         s3 = s1 + q + 1
         return (s1 + s2 + s3) / 3.0 * F
     if name == "euler_dss":
-        # qdp_pre(Q) + fdss_pre in, Qdp(Q) + fdss out; stage 3 also reads Qdp(n0)(Q) for the average
+        # two launches (stages 2 and 3): qdp_pre(Q) + fdss_pre in, Qdp(Q) + fdss out; stage 3 also
+        # reads Qdp(n0)(Q) for the average
         s1 = 2 * q + 2
-        return (3 * s1 + q) / 3.0 * F
+        return (2 * s1 + q) / 2.0 * F
     if name == "euler_qmm_lap":
         # Qdp(Q), derived_dp, divdp_proj, dpdiss_ave in; qbih_pre(Q), qlim out
         return (2 * q + 3 + ql) * F
@@ -75,6 +78,19 @@ def kernel_bytes(name, nelem, nlev, q, launches):
     if name == "remap":
         return ((3 + q) * 2 + 1 + q) * F  # v,T,Qdp in/out, dp3d, Q out
     return None
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "preqx_step_ne30_L72_Q40_hbm_traffic.json")
+
+
+def measured_traffic(phase, args):
+    """HBM bytes per launch of a timed phase from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (measured on the default ne30 L72 Q40
+    workload only; None for other sizes or phases)."""
+    if (args.ne, args.nlev, args.qsize) != (30, 72, 40) or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        return json.load(f)["traffic_bytes_per_launch"].get(phase)
 
 
 def progress(msg):
@@ -248,7 +264,7 @@ def main():
         byts = kernel_bytes(dom, nelem, nlev, q, cnt)
         ach = byts / avg_s / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": byts,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, args), "bytes_per_launch": byts,
                 "avg_launch_ms": round(avg_s * 1e3, 4), "launches": cnt}
     model = step_model_bytes(nelem, nlev, q)
     step_roof = {"model_bytes_per_step": model, "achieved": round(model / sec / 1e9, 1),
