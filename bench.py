@@ -153,9 +153,9 @@ def run_cpu(prob, min_seconds=10.0, max_steps=20):
     return (t1 - t0) / n, n, threads
 
 
-def run_adjoint(steps=10, traj_steps=48, every=8):
+def run_adjoint(rsplit, steps=10, traj_steps=48, every=8):
     """Cost of the differentiable form on the ne16 L72 Q4 workload (dry,
-    rsplit = 0, dt = 300 s): one adjoint step (hommexx_hip_adjoint_step,
+    dt = 300 s, ``rsplit`` 0 or 1): one adjoint step (hommexx_hip_adjoint_step,
     which recomputes the step's stage states and runs the reverse sweep)
     against one forward step (prim_run_subcycle_c), both timed over
     ``steps`` calls after a warm-up; and a checkpointed 48-step trajectory,
@@ -165,8 +165,9 @@ def run_adjoint(steps=10, traj_steps=48, every=8):
                                      synthetic_target, trajectory)
     from hommexx_amd.problem import build_problem
     nu = 7e15
-    prob = build_problem(ne=16, nlev=72, qsize=4, seed=32, rsplit=0, nu=nu, nu_p=nu, nu_q=nu, nu_s=nu, nu_div=nu)
-    progress("adjoint: ne16 L72 Q4 rsplit=0, nelem %d" % prob.nelem)
+    prob = build_problem(ne=16, nlev=72, qsize=4, seed=32, rsplit=rsplit, nu=nu, nu_p=nu, nu_q=nu, nu_s=nu,
+                         nu_div=nu)
+    progress("adjoint: ne16 L72 Q4 rsplit=%d, nelem %d" % (rsplit, prob.nelem))
     with DiffSession(prob) as s:
         s.step(2)
         s.h.synchronize()
@@ -196,8 +197,9 @@ def run_adjoint(steps=10, traj_steps=48, every=8):
         torch.cuda.synchronize()
         t2 = time.perf_counter()
     progress("adjoint: trajectory forward %.3f s, backward %.3f s" % (t1 - t0, t2 - t1))
-    return {"config": {"workload": "preqx step ne16 np4 L72 Q4, dt=300s, rsplit=0 (Eulerian vertical), dry, "
-                                   "hv subcycle 3, nu=7e15", "nelem": prob.nelem},
+    vert = "rsplit=0 (Eulerian vertical)" if rsplit == 0 else "rsplit=1 (PPM remap of the states every step)"
+    return {"config": {"workload": "preqx step ne16 np4 L72 Q4, dt=300s, %s, dry, hv subcycle 3, nu=7e15" % vert,
+                       "nelem": prob.nelem},
             "forward_step_ms": round(fwd * 1e3, 3), "adjoint_step_ms": round(adj * 1e3, 3),
             "adjoint_over_forward": round(adj / fwd, 3),
             "trajectory": {"steps": traj_steps, "checkpoint_every": every,
@@ -252,7 +254,11 @@ def main():
                  "speedup_vs_fp64": round(sec / msec, 4), "max_rel_l2_vs_fp64": max(err.values()),
                  "rel_l2_vs_fp64": err, "steps_compared": args.warmup + args.steps,
                  "dry_mass_rel_drift": mdry, "tracer_mass_rel_drift_max": mq}
-    adjoint = None if args.no_adjoint else run_adjoint()
+    adjoint = None
+    if not args.no_adjoint:
+        a1, a0 = run_adjoint(1), run_adjoint(0)
+        adjoint = dict(a1)
+        adjoint["rsplit0"] = a0
     sypd = args.dt / (365.0 * sec)
     # dominant kernel by total time
     kern = {k: v for k, v in prof.items() if kernel_bytes(k, nelem, nlev, q, 1) is not None}

@@ -5,13 +5,16 @@ The forward pass runs ``prim_run_subcycle_c`` (the HIP step) and keeps the
 prognostic state (v, T, ps) every ``every`` steps as device checkpoints.  The
 backward pass replays each segment from its checkpoint, keeping the input
 state of every step of the segment, and runs ``hommexx_hip_adjoint_step``
-(hand-written HIP adjoint kernels of CAAR, hyperviscosity and DSS) over the
+(hand-written HIP adjoint kernels of CAAR, hyperviscosity, DSS and the
+PPM remap) over the
 segment's steps in reverse.  Device memory: ``ceil(nsteps/every)``
 checkpoints plus ``every`` step states, each (v, T, ps).
 
-The adjoint covers the dry configuration with ``rsplit = 0``: there the
-vertical remap moves only tracers, which do not act on the dynamics, so the
-gradient with respect to the initial (v, T, ps) needs no remap adjoint.
+The adjoint covers dry configurations with ``rsplit`` 0 or 1.  With
+``rsplit = 1`` the states are remapped every step and the backward pass runs
+the linearised PPM remap (branches frozen from the forward pass); with
+``rsplit = 0`` the remap moves only tracers.  Tracers do not act on the
+dynamics of a dry run, so they have no adjoint.
 
 State tensors use the library's device layouts: v ``(nelem, nlev, 2, 4, 4)``,
 T ``(nelem, nlev, 4, 4)``, ps ``(nelem, 4, 4)``, float64 on the GPU.
@@ -35,12 +38,12 @@ def _ptr(t):
 
 class DiffSession:
     """A :class:`HipSession` with device-side state access and the adjoint
-    step.  ``prob.params`` must describe a dry ``rsplit = 0`` run."""
+    step.  ``prob.params`` must describe a dry run with ``rsplit`` 0 or 1."""
 
     def __init__(self, prob, device="cuda"):
         p = prob.params
-        if p.rsplit != 0 or p.moisture or p.qsplit != 1 or p.hypervis_scaling != 0.0 or p.ftype != -1:
-            raise HommexxError("the adjoint needs rsplit=0, qsplit=1, no moisture, constant HV and no forcing")
+        if p.rsplit > 1 or p.moisture or p.qsplit != 1 or p.hypervis_scaling != 0.0 or p.ftype != -1:
+            raise HommexxError("the adjoint needs rsplit=0 or 1, qsplit=1, no moisture, constant HV and no forcing")
         self.prob = prob
         self.device = torch.device(device)
         self.h = HipSession(prob)

@@ -1,11 +1,10 @@
-// Adjoint (reverse-mode derivative) of the dry preqx dynamics step with
-// rsplit = 0.  With rsplit = 0 the vertical coordinate is Eulerian, the
-// vertical remap moves only tracers, and the step maps (v, T, ps) at n0 to
-// (v, T, ps) at np1 through
+// Adjoint (reverse-mode derivative) of the dry preqx dynamics step, rsplit
+// 0 or 1.  The step maps (v, T, ps) at n0 to (v, T, ps) at np1 through
 //     dp3d = dhyai*ps0 + dhybi*ps                      (prim_driver.cpp:101-110)
 //     five CAAR stages + DSS, RK5 combination          (prim_advance_exp.cpp:113-161)
 //     hypervis_subcycle biharmonic cycles              (HyperviscosityFunctorImpl.hpp)
-//     ps = hyai(0)*ps0 + sum_k dp3d                    (rsplit = 0 branch of the remap)
+//     rsplit = 0: ps = hyai(0)*ps0 + sum_k dp3d; the remap moves only tracers
+//     rsplit = 1: PPM remap of v, T to the hybrid levels of that ps
 // Tracers do not act on the dynamics of a dry run, so they have no adjoint
 // here.  The kernels below are the transposes of the forward kernels' linear
 // maps, evaluated at the forward states the driver recomputes:
@@ -188,8 +187,9 @@ __global__ __launch_bounds__(256) void adj_caar_b1_kernel(AdjCaarArgs a) {
   // vertical advection: ta(j) = facp(j)*(f(j+1)-f(j)) + facm(j)*(f(j)-f(j-1)),
   // facp(j) = 0.5/dp(j)*eta(j+1), facm(j) = 0.5/dp(j)*eta(j), eta(0) = eta(L) = 0
   const double stot = a.stot[(size_t)ie * 16 + pt];
-  const double eta_k = k == 0 ? 0.0 : a.hybi[k] * stot - C;
-  const double eta_k1 = k + 1 >= L ? 0.0 : a.hybi[k + 1] * stot - a.c[sidx(L, ie, k + 1, pt)];
+  const bool rs0 = a.rsplit0;  // rsplit > 0: Lagrangian levels, no vertical flux
+  const double eta_k = k == 0 || !rs0 ? 0.0 : a.hybi[k] * stot - C;
+  const double eta_k1 = k + 1 >= L || !rs0 ? 0.0 : a.hybi[k + 1] * stot - a.c[sidx(L, ie, k + 1, pt)];
   const double facp = 0.5 / dp * eta_k1, facm = 0.5 / dp * eta_k;
   // adjoint of the T and v vertical-advection terms: la = -(h sph lambda_pre)
   const double laT = -aT, la0 = -au0, la1 = -au1;
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(256) void adj_caar_b1_kernel(AdjCaarArgs a) {
   a.lv[vidx(L, ie, k, 0, pt)] = lu0;
   a.lv[vidx(L, ie, k, 1, pt)] = lu1;
   a.ldp[i] = -0.5 * eta_k1 / (dp * dp) * lfacp - 0.5 * eta_k / (dp * dp) * lfacm;
-  a.etab[i] = k == 0 ? 0.0 : 0.5 / dp_m * lfacp_m + 0.5 / dp * lfacm + adp - adp_m;
+  a.etab[i] = k == 0 || !rs0 ? 0.0 : 0.5 / dp_m * lfacp_m + 0.5 / dp * lfacm + adp - adp_m;
   a.divb[i] = -adp;
 }
 
@@ -416,6 +416,350 @@ __global__ void adj_zero_kernel(size_t n, double *__restrict__ p) {
   if (i < n) p[i] = 0.0;
 }
 
+// ----------------------------------------------------------- remap adjoint
+// Adjoint of the PPM remap of the states v (two components) and T for
+// rsplit > 0 (PpmRemap.hpp:288-673; the forward is remap_kernel): extensive
+// values f*dp are remapped from the source thicknesses dp (the state after
+// hyperviscosity) to the hybrid levels of ps = hyai(0) ps0 + sum dp and
+// divided by the target thicknesses.  The linearisation freezes every
+// branch of the forward pass (the target search kid, the dma selection and
+// the parabola adjustments): they are recomputed here from the same values.
+// Scratch per column: REMAP_GRID arrays shared by the fields and
+// REMAP_FIELD arrays per field, each nlev+4 long, stored [array][lev][col].
+enum { RG_TG, RG_PIO, RG_PIN, RG_KID, RG_Z2 };
+enum { RF_AA, RF_DMA, RF_AI, RF_LA, RF_LDMA, RF_LAI, RF_LC0, RF_LC1, RF_LC2, RF_LMASS, RF_LDX, RF_LD, RF_LPIO,
+       RF_LPIN, RF_LTG };
+
+struct RCol {
+  const AdjRemapArgs *a;
+  size_t col, ncol, W;
+  __device__ double &g(int arr, int i) const { return a->scr[((size_t)arr * W + i) * ncol + col]; }
+  __device__ double &f(int fld, int arr, int i) const {
+    return a->scr[(size_t)REMAP_GRID * W * ncol + (((size_t)arr * 3 + fld) * W + i) * ncol + col];
+  }
+};
+
+// grid factors of PpmRemap.hpp:367-413 at j from X = dx(j..j+3), with their
+// partial derivatives dG[m] = dG/dX[m]
+__device__ __forceinline__ double ppm_g(int i, const double X[4], double dG[4]) {
+  const double X0 = X[0], X1 = X[1], X2 = X[2], X3 = X[3];
+  dG[0] = dG[1] = dG[2] = dG[3] = 0.0;
+  double N, Dn, S;
+  switch (i) {
+    case 0:
+      S = X0 + X1 + X2;
+      dG[0] = -X1 / (S * S); dG[1] = (X0 + X2) / (S * S); dG[2] = -X1 / (S * S);
+      return X1 / S;
+    case 1:
+      N = 2.0 * X0 + X1; Dn = X1 + X2;
+      dG[0] = 2.0 / Dn; dG[1] = 1.0 / Dn - N / (Dn * Dn); dG[2] = -N / (Dn * Dn);
+      return N / Dn;
+    case 2:
+      N = X1 + 2.0 * X2; Dn = X0 + X1;
+      dG[0] = -N / (Dn * Dn); dG[1] = 1.0 / Dn - N / (Dn * Dn); dG[2] = 2.0 / Dn;
+      return N / Dn;
+    case 3:
+      Dn = X1 + X2;
+      dG[1] = X2 / (Dn * Dn); dG[2] = -X1 / (Dn * Dn);
+      return X1 / Dn;
+    case 4: {
+      const double g4 = 1.0 / (X0 + X1 + X2 + X3);
+      dG[0] = dG[1] = dG[2] = dG[3] = -g4 * g4;
+      return g4;
+    }
+    case 5:
+      Dn = X1 + X2;
+      dG[1] = 2.0 * X2 * X2 / (Dn * Dn); dG[2] = 2.0 * X1 * X1 / (Dn * Dn);
+      return 2.0 * X1 * X2 / Dn;
+    case 6:
+      N = X0 + X1; Dn = 2.0 * X1 + X2;
+      dG[0] = 1.0 / Dn; dG[1] = 1.0 / Dn - 2.0 * N / (Dn * Dn); dG[2] = -N / (Dn * Dn);
+      return N / Dn;
+    case 7:
+      N = X3 + X2; Dn = 2.0 * X2 + X1;
+      dG[1] = -N / (Dn * Dn); dG[2] = 1.0 / Dn - 2.0 * N / (Dn * Dn); dG[3] = 1.0 / Dn;
+      return N / Dn;
+    case 8:
+      N = X1 * (X0 + X1); Dn = 2.0 * X1 + X2;
+      dG[0] = X1 / Dn; dG[1] = (X0 + 2.0 * X1) / Dn - 2.0 * N / (Dn * Dn); dG[2] = -N / (Dn * Dn);
+      return N / Dn;
+    default:
+      N = X2 * (X2 + X3); Dn = X1 + 2.0 * X2;
+      dG[1] = -N / (Dn * Dn); dG[2] = (2.0 * X2 + X3) / Dn - 2.0 * N / (Dn * Dn); dG[3] = X2 / Dn;
+      return N / Dn;
+  }
+}
+
+// dx(i) of the mirrored ghost layout, i = 0..L+3 is Fortran cell i-1
+__device__ __forceinline__ int ghost_src(int i, int L) {
+  return i == 0 ? 1 : i == 1 ? 0 : i <= L + 1 ? i - 2 : i == L + 2 ? L - 1 : L - 2;
+}
+
+// grid of one column: target thicknesses, partitions, target search
+__global__ __launch_bounds__(256) void adj_remap_grid_kernel(AdjRemapArgs a) {
+  const size_t ncol = (size_t)a.nelem * 16;
+  const size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  const int L = a.nlev, ie = (int)(col >> 4), pt = (int)(col & 15);
+  const RCol c{&a, col, ncol, (size_t)L + 4};
+  auto src = [&](int k) { return a.dp[((size_t)ie * L + k) * 16 + pt]; };
+  double ps = 0.0;
+  for (int k = 0; k < L; ++k) ps += src(k);
+  ps += a.hyai[0] * a.ps0;
+  double acc = 0.0, accn = 0.0;
+  for (int k = 0; k < L; ++k) {
+    const double tg = (a.hyai[k + 1] - a.hyai[k]) * a.ps0 + (a.hybi[k + 1] - a.hybi[k]) * ps;
+    c.g(RG_TG, k) = tg;
+    c.g(RG_PIO, k) = acc;
+    c.g(RG_PIN, k) = accn;
+    acc += src(k);
+    accn += tg;
+  }
+  const double pioL = c.g(RG_PIO, L - 1) + src(L - 1);
+  c.g(RG_PIO, L) = pioL;
+  c.g(RG_PIO, L + 1) = pioL + 1.0;
+  c.g(RG_PIN, L) = pioL;
+  for (int k = 0; k < L; ++k) {
+    const double pk = c.g(RG_PIN, k + 1);
+    int kk = k + 1;
+    while (kk - 1 <= L + 1 && c.g(RG_PIO, kk - 1) <= pk) kk++;
+    kk--;
+    if (kk == L + 1) kk = L;
+    // kid >= 0 for every valid column (pio(0) = 0 <= pin(1)); the clamp keeps
+    // the reads in bounds for a column the forward step rejected (NaN)
+    const int kid = kk - 1 < 0 ? 0 : kk - 1;
+    c.g(RG_KID, k) = (double)kid;
+    c.g(RG_Z2, k) = (pk - (c.g(RG_PIO, kid) + c.g(RG_PIO, kid + 1)) * 0.5) / src(kid);
+  }
+}
+
+// one lane = (column, field): forward recompute of the field's PPM
+// reconstruction, then the reverse sweep; per-field grid adjoints go to
+// scratch for adj_remap_finish_kernel
+__global__ __launch_bounds__(256) void adj_remap_field_kernel(AdjRemapArgs a) {
+  const size_t ncol = (size_t)a.nelem * 16;
+  const size_t lane = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane >= 3 * ncol) return;
+  const int fld = (int)(lane / ncol);
+  const size_t col = lane % ncol;
+  const int L = a.nlev, ie = (int)(col >> 4), pt = (int)(col & 15);
+  const RCol c{&a, col, ncol, (size_t)L + 4};
+  auto src = [&](int k) { return a.dp[((size_t)ie * L + k) * 16 + pt]; };
+  auto DX = [&](int i) { return src(ghost_src(i, L)); };
+  auto fidx = [&](int k) {
+    return fld < 2 ? (((size_t)ie * L + k) * 2 + fld) * 16 + pt : ((size_t)ie * L + k) * 16 + pt;
+  };
+  const double *fin = fld < 2 ? a.v : a.t;
+  double *lam = fld < 2 ? a.l_v : a.l_t;
+  auto F = [&](int arr, int i) -> double & { return c.f(fld, arr, i); };
+  auto A = [&](int j) -> double & { return F(RF_AA, j + 1); };   // a(j), j = -1..L+2
+  auto LA = [&](int j) -> double & { return F(RF_LA, j + 1); };
+  const int W = L + 4;
+  for (int i = 0; i < W; ++i) {
+    F(RF_LA, i) = 0.0; F(RF_LDMA, i) = 0.0; F(RF_LAI, i) = 0.0; F(RF_LC0, i) = 0.0; F(RF_LC1, i) = 0.0;
+    F(RF_LC2, i) = 0.0; F(RF_LMASS, i) = 0.0; F(RF_LDX, i) = 0.0; F(RF_LD, i) = 0.0; F(RF_LPIO, i) = 0.0;
+    F(RF_LPIN, i) = 0.0; F(RF_LTG, i) = 0.0;
+  }
+  // ---------------------------------------------------------- forward
+  for (int k = 0; k < L; ++k) {
+    const double x = src(k);
+    A(k + 1) = (fin[fidx(k)] * x) / x;
+  }
+  A(0) = A(1); A(-1) = A(2); A(L + 1) = A(L); A(L + 2) = A(L - 1);
+  double X[4], dG[4];
+  for (int j = 0; j < L + 2; ++j) {
+    for (int m = 0; m < 4; ++m) X[m] = DX(j + m < L + 3 ? j + m : L + 3);
+    const double g0 = ppm_g(0, X, dG), g1 = ppm_g(1, X, dG), g2 = ppm_g(2, X, dG);
+    const double dp_ = A(j + 1) - A(j), dm_ = A(j) - A(j - 1);
+    double dma = 0.0;
+    if (dp_ * dm_ > 0.0) {
+      const double da = g0 * (g1 * dp_ + g2 * dm_);
+      const double bc = 2.0 * fabs(dm_) < 2.0 * fabs(dp_) ? 2.0 * fabs(dm_) : 2.0 * fabs(dp_);
+      dma = (fabs(da) < bc ? fabs(da) : bc) * copysign(1.0, da);
+    }
+    F(RF_DMA, j) = dma;
+  }
+  for (int j = 0; j < L + 1; ++j) {
+    for (int m = 0; m < 4; ++m) X[m] = DX(j + m);
+    double gg[10];
+    for (int i = 3; i < 10; ++i) gg[i] = ppm_g(i, X, dG);
+    const double d1 = A(j + 1) - A(j);
+    F(RF_AI, j) = A(j) + gg[3] * d1 +
+                  gg[4] * (gg[5] * (gg[6] - gg[7]) * d1 - gg[8] * F(RF_DMA, j + 1) + gg[9] * F(RF_DMA, j));
+  }
+  // parabola of cell j (1..L): flags and coefficients
+  auto parabola = [&](int j, double &c0, double &c1, double &c2, bool &flat, bool &f2, bool &f3, bool &bnd) {
+    const double al0 = F(RF_AI, j - 1), ar0 = F(RF_AI, j), aa = A(j);
+    flat = (ar0 - aa) * (aa - al0) <= 0.0;
+    double al = flat ? aa : al0, ar = flat ? aa : ar0;
+    f2 = (ar - al) * (aa - (al + ar) / 2.0) > (ar - al) * (ar - al) / 6.0;
+    if (f2) al = 3.0 * aa - 2.0 * ar;
+    f3 = (ar - al) * (aa - (al + ar) / 2.0) < -(ar - al) * (ar - al) / 6.0;
+    if (f3) ar = 3.0 * aa - 2.0 * al;
+    c0 = 1.5 * aa - (al + ar) / 4.0;
+    c1 = ar - al;
+    c2 = 3.0 * (-2.0 * aa + (al + ar));
+    bnd = a.alg == 2 && (j <= 2 || j >= L - 1);
+    if (bnd) { c0 = aa; c1 = 0.0; c2 = 0.0; }
+  };
+  // ------------------------------------------ reverse: targets (ascending)
+  double prefix = 0.0, mprev = 0.0;
+  int jj = 0;
+  for (int k = 0; k < L; ++k) {
+    const int kk = (int)c.g(RG_KID, k);
+    while (jj < kk) { prefix += fin[fidx(jj)] * src(jj); ++jj; }
+    const double z2 = c.g(RG_Z2, k), dxk = src(kk), tg = c.g(RG_TG, k);
+    double c0, c1, c2; bool fl, f2, f3, bd;
+    parabola(kk + 1, c0, c1, c2, fl, f2, f3, bd);
+    const double integ = (c0 * (z2 + 0.5) + c1 * (z2 * z2 - 0.25) / 2.0) + c2 * (z2 * z2 * z2 + 0.125) / 3.0;
+    const double m = prefix + integ * dxk;
+    const double out = k > 0 ? m - mprev : m;
+    mprev = m;
+    const double lf = lam[fidx(k)];
+    F(RF_LTG, k) += -lf * (out / tg) / tg;
+    const double lnext = k + 1 < L ? lam[fidx(k + 1)] / c.g(RG_TG, k + 1) : 0.0;
+    const double lm = lf / tg - lnext;
+    F(RF_LMASS, kk) += lm;
+    const double li = lm * dxk;
+    F(RF_LDX, kk) += lm * integ;
+    F(RF_LC0, kk) += li * (z2 + 0.5);
+    F(RF_LC1, kk) += li * (z2 * z2 - 0.25) / 2.0;
+    F(RF_LC2, kk) += li * (z2 * z2 * z2 + 0.125) / 3.0;
+    const double lz2 = li * (c0 + c1 * z2 + c2 * z2 * z2);
+    F(RF_LPIN, k + 1) += lz2 / dxk;
+    F(RF_LPIO, kk) -= 0.5 * lz2 / dxk;
+    F(RF_LPIO, kk + 1) -= 0.5 * lz2 / dxk;
+    F(RF_LDX, kk) -= lz2 * z2 / dxk;
+  }
+  // mass prefix -> lambda of var: LMASS(j) becomes sum_{i > j} LMASS(i)
+  {
+    double s = F(RF_LMASS, L);
+    for (int j = L - 1; j >= 0; --j) {
+      const double o = F(RF_LMASS, j);
+      F(RF_LMASS, j) = s;
+      s += o;
+    }
+  }
+  // cells (descending): coefficients -> a(j), ai(j-1), ai(j)
+  for (int j = L; j >= 1; --j) {
+    const int cidx = j - 1;
+    double c0, c1, c2; bool flat, f2, f3, bnd;
+    parabola(j, c0, c1, c2, flat, f2, f3, bnd);
+    const double l0 = F(RF_LC0, cidx), l1 = F(RF_LC1, cidx), l2 = F(RF_LC2, cidx);
+    if (bnd) { LA(j) += l0; continue; }
+    double laa = 1.5 * l0 - 6.0 * l2;
+    double lal = -l0 / 4.0 - l1 + 3.0 * l2;
+    double lar = -l0 / 4.0 + l1 + 3.0 * l2;
+    if (f3) { laa += 3.0 * lar; lal += -2.0 * lar; lar = 0.0; }
+    if (f2) { laa += 3.0 * lal; lar += -2.0 * lal; lal = 0.0; }
+    if (flat) { laa += lal + lar; lal = 0.0; lar = 0.0; }
+    F(RF_LAI, j - 1) += lal;
+    F(RF_LAI, j) += lar;
+    LA(j) += laa;
+  }
+  // interfaces ai(j), j = L..0
+  for (int j = L; j >= 0; --j) {
+    for (int m = 0; m < 4; ++m) X[m] = DX(j + m);
+    double gg[10], dd[10][4];
+    for (int i = 3; i < 10; ++i) gg[i] = ppm_g(i, X, dd[i]);
+    const double li = F(RF_LAI, j);
+    const double d1 = A(j + 1) - A(j);
+    const double g56 = gg[5] * (gg[6] - gg[7]);
+    const double dmaj = F(RF_DMA, j), dmaj1 = F(RF_DMA, j + 1);
+    LA(j) += li;
+    const double ld1 = li * (gg[3] + gg[4] * g56);
+    LA(j + 1) += ld1;
+    LA(j) -= ld1;
+    F(RF_LDMA, j + 1) -= li * gg[4] * gg[8];
+    F(RF_LDMA, j) += li * gg[4] * gg[9];
+    double lg[10] = {0};
+    lg[3] = li * d1;
+    lg[4] = li * (g56 * d1 - gg[8] * dmaj1 + gg[9] * dmaj);
+    const double lg56 = li * gg[4] * d1;
+    lg[5] = lg56 * (gg[6] - gg[7]);
+    lg[6] = lg56 * gg[5];
+    lg[7] = -lg56 * gg[5];
+    lg[8] = -li * gg[4] * dmaj1;
+    lg[9] = li * gg[4] * dmaj;
+    for (int i = 3; i < 10; ++i)
+      for (int m = 0; m < 4; ++m) F(RF_LD, j + m) += lg[i] * dd[i][m];
+  }
+  // dma(j), j = L+1..0
+  for (int j = L + 1; j >= 0; --j) {
+    for (int m = 0; m < 4; ++m) X[m] = DX(j + m < L + 3 ? j + m : L + 3);
+    double d0[4], d1[4], d2[4];
+    const double g0 = ppm_g(0, X, d0), g1 = ppm_g(1, X, d1), g2 = ppm_g(2, X, d2);
+    const double dp_ = A(j + 1) - A(j), dm_ = A(j) - A(j - 1);
+    if (!(dp_ * dm_ > 0.0)) continue;
+    const double da = g0 * (g1 * dp_ + g2 * dm_);
+    const bool bsel = 2.0 * fabs(dm_) < 2.0 * fabs(dp_);
+    const double bc = bsel ? 2.0 * fabs(dm_) : 2.0 * fabs(dp_);
+    const bool asel = fabs(da) < bc;
+    const double sda = da > 0.0 ? 1.0 : da < 0.0 ? -1.0 : 0.0;
+    const double lm3 = F(RF_LDMA, j) * sda;
+    double lda = 0.0, ldp = 0.0, ldm = 0.0;
+    if (asel) {
+      lda = lm3 * sda;
+    } else if (bsel) {
+      ldm = 2.0 * (dm_ > 0.0 ? 1.0 : dm_ < 0.0 ? -1.0 : 0.0) * lm3;
+    } else {
+      ldp = 2.0 * (dp_ > 0.0 ? 1.0 : dp_ < 0.0 ? -1.0 : 0.0) * lm3;
+    }
+    const double lg0 = lda * (g1 * dp_ + g2 * dm_), lg1 = lda * g0 * dp_, lg2 = lda * g0 * dm_;
+    ldp += lda * g0 * g1;
+    ldm += lda * g0 * g2;
+    LA(j + 1) += ldp;
+    LA(j) -= ldp;
+    LA(j) += ldm;
+    LA(j - 1) -= ldm;
+    for (int m = 0; m < 3; ++m) {
+      const int i = j + m < L + 3 ? j + m : L + 3;
+      F(RF_LD, i) += lg0 * d0[m] + lg1 * d1[m] + lg2 * d2[m];
+    }
+  }
+  // ghosts of a fold back; a = var/dx, var = f*dx
+  for (int k = 0; k < L; ++k) {
+    double lao = LA(k + 1);
+    if (k == 0) lao += LA(0);
+    if (k == 1) lao += LA(-1);
+    if (k == L - 1) lao += LA(L + 1);
+    if (k == L - 2) lao += LA(L + 2);
+    const double x = src(k), fv = fin[fidx(k)], var = fv * x;
+    const double lvar = F(RF_LMASS, k) + lao / x;
+    F(RF_LDX, k) += -lao * var / (x * x) + lvar * fv;
+    lam[fidx(k)] = lvar * x;
+  }
+}
+
+// per column: the fields' grid adjoints -> adjoint of dp (source thickness)
+__global__ __launch_bounds__(256) void adj_remap_finish_kernel(AdjRemapArgs a) {
+  const size_t ncol = (size_t)a.nelem * 16;
+  const size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  const int L = a.nlev, ie = (int)(col >> 4), pt = (int)(col & 15);
+  const RCol c{&a, col, ncol, (size_t)L + 4};
+  auto sum3 = [&](int arr, int i) { return c.f(0, arr, i) + c.f(1, arr, i) + c.f(2, arr, i); };
+  // lambda of pio(L) collects pio(L+1) = pio(L) + 1 and pin(L) = pio(L)
+  double spio = sum3(RF_LPIO, L) + sum3(RF_LPIO, L + 1) + sum3(RF_LPIN, L);
+  double spin = 0.0, lps = a.l_ps[col];
+  double *ldp = a.l_dp + (size_t)ie * L * 16 + pt;
+  for (int k = L - 1; k >= 0; --k) {
+    // dx(k) enters pio(i) for i > k, the ghosts and ps
+    double x = sum3(RF_LDX, k) + sum3(RF_LD, k + 2) + spio;
+    if (k == 1) x += sum3(RF_LD, 0);
+    if (k == 0) x += sum3(RF_LD, 1);
+    if (k == L - 1) x += sum3(RF_LD, L + 2);
+    if (k == L - 2) x += sum3(RF_LD, L + 3);
+    ldp[(size_t)k * 16] = x;
+    spio += sum3(RF_LPIO, k);
+    // tg(k) enters pin(i) for k < i <= L-1
+    const double ltg = sum3(RF_LTG, k) + spin;
+    spin += sum3(RF_LPIN, k);
+    lps += (a.hybi[k + 1] - a.hybi[k]) * ltg;
+  }
+  for (int k = 0; k < L; ++k) ldp[(size_t)k * 16] += lps;
+}
+
 inline unsigned nblk(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
@@ -446,6 +790,16 @@ void launch_adj_fill_dp(hipStream_t st, int nelem, int nlev, const double *lps, 
 void launch_adj_ps(hipStream_t st, int nelem, int nlev, const double *hybi, const double *ldp, double *lps) {
   hipLaunchKernelGGL(adj_ps_kernel, dim3(nblk((size_t)nelem * 16)), dim3(256), 0, st, nelem, nlev, hybi, ldp, lps);
 }
+void launch_adj_remap(hipStream_t st, const AdjRemapArgs &a) {
+  const size_t ncol = (size_t)a.nelem * 16;
+  hipLaunchKernelGGL(adj_remap_grid_kernel, dim3(nblk(ncol)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(adj_remap_field_kernel, dim3(nblk(3 * ncol)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(adj_remap_finish_kernel, dim3(nblk(ncol)), dim3(256), 0, st, a);
+}
+size_t adj_remap_scratch(int nelem, int nlev) {
+  return (size_t)(REMAP_GRID + 3 * REMAP_FIELD) * (nlev + 4) * nelem * 16;
+}
+
 void launch_adj_zero(hipStream_t st, size_t n, double *p) {
   hipLaunchKernelGGL(adj_zero_kernel, dim3(nblk(n)), dim3(256), 0, st, n, p);
 }

@@ -616,6 +616,7 @@ struct AdjBufs {
   std::vector<StateBuf> U, H;  // CAAR stage inputs U0..U4, HV cycle states H0..Hnsub
   StateBuf W{}, LX{}, L0{}, L1{}, LP{}, S{}, M{};
   double *div, *p, *cc, *stot, *pb, *phib, *etab, *ombp, *divb, *lv, *lt, *ldp;
+  double *rscr = nullptr;  // remap adjoint scratch (rsplit > 0)
   double *base = nullptr;
 };
 AdjBufs *adjb = nullptr;
@@ -639,7 +640,8 @@ bool adj_alloc() {
   if (!adjb) adjb = new AdjBufs();
   AdjBufs &b = *adjb;
   const int nstates = 5 + 1 + (nsub + 1) + 6;  // U, W, H, LX L0 L1 LP S M
-  const size_t total = (size_t)nstates * 4 * f3 + 12 * f3 + (size_t)c.nelem * 16;
+  const size_t nrs = adj_remap_scratch(c.nelem, c.nlev);
+  const size_t total = (size_t)nstates * 4 * f3 + 12 * f3 + (size_t)c.nelem * 16 + nrs;
   b.base = dalloc<double>(total);  // owned by the session (freed with it)
   if (!b.base) return false;
   b.f3 = f3;
@@ -664,6 +666,8 @@ bool adj_alloc() {
   b.lv = q;
   q += 2 * f3;
   b.stot = q;
+  q += (size_t)c.nelem * 16;
+  b.rscr = q;
   return true;
 }
 
@@ -708,6 +712,7 @@ void caar_adjoint(const StateBuf &xb, const StateBuf &lam, double h, const State
   a.hybi = c.hybi_d;
   a.h = h;
   a.p_top = c.hyai0 * c.ps0;
+  a.rsplit0 = c.p.rsplit == 0;
   a.div = b.div;
   a.p = b.p;
   a.c = b.cc;
@@ -1333,8 +1338,8 @@ int hommexx_hip_set_state(const double *v, const double *T, const double *ps) {
   return hipok(hipStreamSynchronize(c.st), "hommexx_hip_set_state") ? 0 : 2;
 }
 
-// Adjoint of one step of length dt (one prim_run_subcycle_c call) of the dry
-// rsplit = 0 configuration, taken at the step's input state (v, T, ps):
+// Adjoint of one step of length dt (one prim_run_subcycle_c call) of a dry
+// configuration with rsplit = 0 or 1, taken at the step's input state (v, T, ps):
 // on entry lam_* hold the adjoint of the step's output (v, T, ps), on return
 // the adjoint of its input.  All pointers are device memory in the layouts of
 // hommexx_hip_get_state.  The model state of the session is not changed.
@@ -1342,8 +1347,8 @@ int hommexx_hip_adjoint_step(double dt, const double *v, const double *T, const 
                              double *lam_T, double *lam_ps) {
   Ctx &c = C();
   if (!ready("hommexx_hip_adjoint_step")) return 1;
-  if (c.p.rsplit != 0 || c.p.qsplit != 1 || c.p.moist || !c.consthv || c.form != 0 || c.p.ftype != -1) {
-    fail(70, "hommexx_hip_adjoint_step: needs rsplit = 0, qsplit = 1, no moisture, constant-coefficient "
+  if (c.p.rsplit > 1 || c.p.qsplit != 1 || c.p.moist || !c.consthv || c.form != 0 || c.p.ftype != -1) {
+    fail(70, "hommexx_hip_adjoint_step: needs rsplit = 0 or 1, qsplit = 1, no moisture, constant-coefficient "
              "hyperviscosity, no forcing and the FP64 form");
     return 2;
   }
@@ -1375,7 +1380,27 @@ int hommexx_hip_adjoint_step(double dt, const double *v, const double *T, const 
   Scope s("adj_reverse");
   hipMemcpyAsync(b.LX.v, lam_v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
   hipMemcpyAsync(b.LX.t, lam_T, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
-  launch_adj_fill_dp(c.st, c.nelem, c.nlev, lam_ps, b.LX.dp);  // ps = hyai(0) ps0 + sum_k dp3d
+  if (c.p.rsplit == 0) {
+    launch_adj_fill_dp(c.st, c.nelem, c.nlev, lam_ps, b.LX.dp);  // ps = hyai(0) ps0 + sum_k dp3d
+  } else {
+    // linearised PPM remap of v and T from the thicknesses after hyperviscosity
+    AdjRemapArgs r{};
+    r.nelem = c.nelem;
+    r.nlev = c.nlev;
+    r.alg = c.p.remap_alg;
+    r.v = H[nsub].v;
+    r.t = H[nsub].t;
+    r.dp = H[nsub].dp;
+    r.l_v = b.LX.v;
+    r.l_t = b.LX.t;
+    r.l_dp = b.LX.dp;
+    r.l_ps = lam_ps;
+    r.hyai = c.hyai_d;
+    r.hybi = c.hybi_d;
+    r.ps0 = c.ps0;
+    r.scr = b.rscr;
+    launch_adj_remap(c.st, r);
+  }
   for (int cyc = nsub - 1; cyc >= 0; --cyc) hv_adjoint(H[cyc], H[cyc + 1], b.LX, dt / nsub);
   launch_adj_zero(c.st, 4 * f3, b.L0.v);
   launch_adj_zero(c.st, 4 * f3, b.L1.v);

@@ -91,7 +91,7 @@ struct RemapArgs {
   int *error;                 // set to 1 on invalid thickness, 2 on unsupported deformation
 };
 
-// Adjoint of one CAAR stage (dry, rsplit = 0).  The stage is
+// Adjoint of one CAAR stage (dry).  The stage is
 //   x_out = DSS_r( sph*(x_a + h*RHS(x_b)) );
 // given lp = DSS_r(adjoint of x_out), the kernels add the adjoint of x_b into
 // b (overwrite or accumulate) and c1*sph*lp into a1 (and c2*sph*lp into a2).
@@ -102,6 +102,7 @@ struct AdjCaarArgs {
   const double *lp_v, *lp_t, *lp_dp;   // DSS_r of the output adjoint
   const double *hybi;                  // nlev+1
   double h, p_top;
+  int rsplit0;                         // rsplit == 0: Eulerian vertical advection terms
   // scratch (one 3-D scalar each; stot is 2-D)
   double *div, *p, *c, *stot, *pb, *phib, *etab, *ombp, *divb, *lv, *lt, *ldp;
   double *b_v, *b_t, *b_dp;
@@ -123,7 +124,23 @@ struct AdjHvArgs {
   double dt, nu, nu_s, nu_p, nu_top, nu_ratio1, nu_ratio2;
 };
 
+// Adjoint of the PPM remap of v and T (rsplit > 0) from the source
+// thicknesses dp to the hybrid levels of ps = hyai(0) ps0 + sum dp
+constexpr int REMAP_GRID = 5, REMAP_FIELD = 15;
+struct AdjRemapArgs {
+  int nelem, nlev, alg;
+  const double *v, *t, *dp;            // remap input (the state after hyperviscosity)
+  double *l_v, *l_t;                   // in: adjoint of the remapped v, T; out: of the input v, T
+  double *l_dp;                        // out: adjoint of dp
+  const double *l_ps;                  // adjoint of ps(np1)
+  const double *hyai, *hybi;
+  double ps0;
+  double *scr;                         // adj_remap_scratch() doubles
+};
+size_t adj_remap_scratch(int nelem, int nlev);
+
 // launchers (each enqueues on `st`)
+void launch_adj_remap(hipStream_t st, const AdjRemapArgs &a);
 void launch_adj_caar(hipStream_t st, const AdjCaarArgs &a);
 void launch_adj_hv_pre(hipStream_t st, const AdjHvArgs &a);
 void launch_adj_hv_a(hipStream_t st, const AdjHvArgs &a);

@@ -77,21 +77,61 @@ def derivative_matrix():
     return d
 
 
-class CubedSphere:
-    """Geometry and connectivity of an ``ne x ne x 6`` cubed sphere."""
+def _hilbert_index(n, x, y):
+    """Position of cell (x, y) along the Hilbert curve over an n x n grid
+    (n a power of two)."""
+    d = 0
+    s = n // 2
+    while s > 0:
+        rx = 1 if x & s else 0
+        ry = 1 if y & s else 0
+        d += s * s * ((3 * rx) ^ ry)
+        if ry == 0:
+            if rx == 1:
+                x, y = s - 1 - x, s - 1 - y
+            x, y = y, x
+        s //= 2
+    return d
 
-    def __init__(self, ne):
+
+def sfc_order(ne):
+    """Face-major element order with each face's ne x ne elements along a
+    Hilbert curve (cells of the enclosing power-of-two grid outside the face
+    are skipped), like HOMME's space-filling-curve element ordering
+    (src/share/spacecurve_mod.F90): elements that share an edge are mostly
+    close in the order.  Returns the natural (face, ej, ei) row-major index
+    of each position."""
+    n = 1
+    while n < ne:
+        n *= 2
+    face = sorted((_hilbert_index(n, i, j), j * ne + i) for j in range(ne) for i in range(ne))
+    return np.array([f * ne * ne + c for f in range(6) for _, c in face], dtype=np.int64)
+
+
+class CubedSphere:
+    """Geometry and connectivity of an ``ne x ne x 6`` cubed sphere.
+
+    ``order``: "sfc" (default) numbers the elements along a space-filling
+    curve on each face, as HOMME does; "rowmajor" numbers them face by face,
+    row by row."""
+
+    def __init__(self, ne, order="sfc"):
         if ne < 2:
             raise ValueError("ne must be >= 2 (each element needs at most one missing corner)")
+        if order not in ("sfc", "rowmajor"):
+            raise ValueError("order must be 'sfc' or 'rowmajor'")
         self.ne = ne
         self.nelem = 6 * ne * ne
+        self.order = sfc_order(ne) if order == "sfc" else np.arange(self.nelem)
+        self._rank = np.empty_like(self.order)
+        self._rank[self.order] = np.arange(self.nelem)
         self._build_geometry()
         self._build_connectivity()
         self._build_mass()
 
     # ------------------------------------------------------------------ geometry
     def elem_index(self, face, ei, ej):
-        return face * self.ne * self.ne + ej * self.ne + ei
+        return int(self._rank[face * self.ne * self.ne + ej * self.ne + ei])
 
     def _build_geometry(self):
         ne = self.ne
@@ -136,6 +176,9 @@ class CubedSphere:
                 Xc = n + np.tan(AA)[..., None] * e1 + np.tan(BB)[..., None] * e2
                 Xc = Xc / np.sqrt(np.sum(Xc * Xc, axis=-1))[..., None]
                 corner3d[sl, k] = Xc.reshape(ne * ne, 3)
+        # element numbering: position i holds the natural element order[i]
+        pos, dpda, dpdb, corner3d = pos[self.order], dpda[self.order], dpdb[self.order], corner3d[self.order]
+        self.face_of = self.face_of[self.order]
         # snap near-zero coordinates (the poles and face centres are nodes when ne is even)
         pos[np.abs(pos) < 1e-14] = 0.0
         self.pos = pos

@@ -7,6 +7,10 @@
   of 1e-6, along 64 seeded directions at ne16 L72 Q4 over 48 steps with a
   checkpoint every 8 steps.
 
+Both dry configurations are covered: rsplit = 0 (Eulerian vertical
+coordinate) and rsplit = 1 (states remapped every step: the backward pass
+runs the linearised PPM remap with the forward pass's branches frozen).
+
 The gradient is taken of a regional kinetic-energy-weighted mismatch against
 a seeded synthetic target, with respect to the initial T and ps (and v).
 """
@@ -23,16 +27,18 @@ from torch_ref import TorchStep
 pytestmark = pytest.mark.gpu
 
 
-def ne16_problem():
+def ne16_problem(rsplit):
     # ne16 hyperviscosity as the reference's ne16 setups use it (nu = 7e15)
     nu = 7e15
-    return build_problem(ne=16, nlev=72, qsize=4, seed=32, rsplit=0, nu=nu, nu_p=nu, nu_q=nu, nu_s=nu, nu_div=nu)
+    return build_problem(ne=16, nlev=72, qsize=4, seed=32, rsplit=rsplit, nu=nu, nu_p=nu, nu_q=nu, nu_s=nu,
+                         nu_div=nu)
 
 
-def test_adjoint_matches_torch_reference_ne4():
-    prob = build_problem(ne=4, nlev=26, qsize=2, seed=31, rsplit=0, t_noise=0.1)
+@pytest.mark.parametrize("rsplit,alg", [(0, 1), (1, 1), (1, 2)])
+def test_adjoint_matches_torch_reference_ne4(rsplit, alg):
+    prob = build_problem(ne=4, nlev=26, qsize=2, seed=31, rsplit=rsplit, remap_alg=alg, t_noise=0.1)
     nsteps, every = 6, 4
-    print("case: ne4 L26 rsplit0, %d steps, checkpoint every %d" % (nsteps, every), flush=True)
+    print("case: ne4 L26 rsplit %d alg %d, %d steps, checkpoint every %d" % (rsplit, alg, nsteps, every), flush=True)
     # pure-PyTorch FP64 reference on the CPU
     x0 = [a.clone().requires_grad_() for a in initial_state(prob, "cpu")]
     out = TorchStep(prob).run(*x0, nsteps)
@@ -51,9 +57,10 @@ def test_adjoint_matches_torch_reference_ne4():
         assert e <= 1e-9, (n, e)
 
 
-def test_adjoint_checkpoint_interval_independent():
+@pytest.mark.parametrize("rsplit", [0, 1])
+def test_adjoint_checkpoint_interval_independent(rsplit):
     """The gradient does not depend on the checkpoint interval."""
-    prob = build_problem(ne=4, nlev=26, qsize=0, seed=33, rsplit=0)
+    prob = build_problem(ne=4, nlev=26, qsize=0, seed=33, rsplit=rsplit)
     tgt, w = synthetic_target(prob, 2), region_weight(prob)
     grads = []
     with DiffSession(prob) as s:
@@ -66,10 +73,11 @@ def test_adjoint_checkpoint_interval_independent():
             assert torch.equal(a, b)
 
 
-def test_adjoint_central_differences_ne16():
-    prob = ne16_problem()
+@pytest.mark.parametrize("rsplit", [0, 1])
+def test_adjoint_central_differences_ne16(rsplit):
+    prob = ne16_problem(rsplit)
     nsteps, every, ndir, eps = 48, 8, 64, 1e-6
-    print("case: ne16 L72 Q4 rsplit0, %d steps, checkpoint every %d, %d directions" % (nsteps, every, ndir),
+    print("case: ne16 L72 Q4 rsplit %d, %d steps, checkpoint every %d, %d directions" % (rsplit, nsteps, every, ndir),
           flush=True)
     tgt, w = synthetic_target(prob, 6), region_weight(prob)
     with DiffSession(prob) as s:

@@ -26,12 +26,15 @@ RRE = 1.0 / 6.376e6
 
 
 class TorchStep:
-    """One dry rsplit=0 step as a differentiable torch function."""
+    """One dry step (rsplit = 0 or 1) as a differentiable torch function."""
 
     def __init__(self, prob, device="cpu"):
         p = prob.params
-        if p.rsplit != 0 or p.moisture or p.hypervis_scaling != 0.0 or p.qsplit != 1:
-            raise ValueError("the torch reference covers dry rsplit=0, qsplit=1, constant-coefficient HV")
+        if p.rsplit > 1 or p.moisture or p.hypervis_scaling != 0.0 or p.qsplit != 1 or p.remap_alg not in (1, 2):
+            raise ValueError("the torch reference covers dry rsplit=0/1, qsplit=1, constant-coefficient HV, "
+                             "remap alg 1/2")
+        self.rsplit = p.rsplit
+        self.remap_alg = p.remap_alg
         t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=device)
         m = prob.metric
         self.L = prob.nlev
@@ -156,6 +159,8 @@ class TorchStep:
         stot = divv.sum(1, keepdim=True)
         eta = self.hybi * stot - excl(divv)
         eta = torch.cat([torch.zeros_like(eta[:, :1]), eta[:, 1:]], 1)  # eta(0) = 0
+        if self.rsplit > 0:
+            eta = torch.zeros_like(eta)  # Lagrangian levels: no vertical flux
         eta_up = torch.cat([eta[:, 1:], torch.zeros_like(eta[:, :1])], 1)  # eta(k+1), 0 at the bottom
         facp = 0.5 * (1 / dp) * eta_up
         facm = 0.5 * (1 / dp) * eta
@@ -229,7 +234,113 @@ class TorchStep:
         for _ in range(self.nsub):
             x = self.hv_cycle(x, dt / self.nsub)
         vn, Tn, dpn = x
-        return vn, Tn, self.ptop + dpn.sum(1)
+        if self.rsplit == 0:
+            return vn, Tn, self.ptop + dpn.sum(1)
+        return self.remap_states(vn, Tn, dpn)
+
+    # ------------------------------------------------------ vertical remap
+    def remap_states(self, v, T, src):
+        """PPM remap of v and T (as extensive v*dp, T*dp) from the Lagrangian
+        thicknesses ``src`` to the hybrid levels of ps = hyai(0) ps0 + sum(src)
+        (PpmRemap.hpp:288-673, the oracle's ppm_grids / ppm_remap_column)."""
+        L = self.L
+        ps = self.ptop + src.sum(1)
+        tgt = self.dA + self.dB * ps.unsqueeze(1)
+        # columns: (ncol, L)
+        col = lambda x: x.permute(0, 2, 3, 1).reshape(-1, L)
+        uncol = lambda x: x.reshape(src.shape[0], 4, 4, L).permute(0, 3, 1, 2)
+        dx, tg = col(src), col(tgt)
+        fields = [col(v[:, :, 0]), col(v[:, :, 1]), col(T)]
+        out = [uncol(self._ppm_column(dx, tg, f * dx) / tg) for f in fields]
+        return torch.stack([out[0], out[1]], 2), out[2], ps
+
+    def _ppm_column(self, dx, tg, var):
+        L = self.L
+        z = torch.zeros_like(dx[:, :1])
+        pio = torch.cat([z, torch.cumsum(dx, 1)], 1)               # pio(0..L)
+        pio = torch.cat([pio, pio[:, -1:] + 1.0], 1)                # pio(L+1)
+        pin = torch.cat([z, torch.cumsum(tg, 1)], 1)                # pin(0..L)
+        pin = torch.cat([pin[:, :L], pio[:, L:L + 1]], 1)           # pin(L) = pio(L)
+        # mirrored ghosts: DX(i), i = 0..L+3, is dx at Fortran cell i-1
+        D = torch.cat([dx[:, 1:2], dx[:, 0:1], dx, dx[:, L - 1:L], dx[:, L - 2:L - 1]], 1)
+        # target search on detached values (kid is piecewise constant)
+        with torch.no_grad():
+            pio_d, pin_d = pio.detach(), pin.detach()
+            kid = torch.zeros(dx.shape[0], L, dtype=torch.long, device=dx.device)
+            for k in range(L):
+                # kk = smallest index >= k+1 with pio(kk-1) > pin(k+1), capped
+                cond = pio_d[:, k:L + 2] <= pin_d[:, k + 1:k + 2]       # pio(kk-1) for kk-1 = k..L+1
+                # count consecutive trues from the start
+                run = torch.cumprod(cond.to(torch.int64), 1).sum(1)
+                kk = k + 1 + run - 1
+                kk = torch.where(kk == L + 1, torch.full_like(kk, L), kk)
+                kid[:, k] = kk - 1
+        rows = torch.arange(dx.shape[0], device=dx.device)
+        DXk = lambda j: D[:, j]
+        # ppmdx coefficients, j = 0..L+1 (and 0..L)
+        G = [[None] * (L + 2) for _ in range(10)]
+        for j in range(L + 2):
+            G[0][j] = DXk(j + 1) / (DXk(j) + DXk(j + 1) + DXk(j + 2))
+            G[1][j] = (2.0 * DXk(j) + DXk(j + 1)) / (DXk(j + 1) + DXk(j + 2))
+            G[2][j] = (DXk(j + 1) + 2.0 * DXk(j + 2)) / (DXk(j) + DXk(j + 1))
+        for j in range(L + 1):
+            G[3][j] = DXk(j + 1) / (DXk(j + 1) + DXk(j + 2))
+            G[4][j] = 1.0 / (DXk(j) + DXk(j + 1) + DXk(j + 2) + DXk(j + 3))
+            G[5][j] = (2.0 * DXk(j + 1) * DXk(j + 2)) / (DXk(j + 1) + DXk(j + 2))
+            G[6][j] = (DXk(j) + DXk(j + 1)) / (2.0 * DXk(j + 1) + DXk(j + 2))
+            G[7][j] = (DXk(j + 3) + DXk(j + 2)) / (2.0 * DXk(j + 2) + DXk(j + 1))
+            G[8][j] = DXk(j + 1) * (DXk(j) + DXk(j + 1)) / (2.0 * DXk(j + 1) + DXk(j + 2))
+            G[9][j] = DXk(j + 2) * (DXk(j + 2) + DXk(j + 3)) / (DXk(j + 1) + 2.0 * DXk(j + 2))
+        ao = var / dx
+        if self.remap_alg == 3:
+            raise ValueError("remap alg 3 is not accepted by the C-ABI")
+        A = torch.cat([ao[:, 1:2], ao[:, 0:1], ao, ao[:, L - 1:L], ao[:, L - 2:L - 1]], 1)  # a(j) = A[:, j+1]
+        a = lambda j: A[:, j + 1]
+        dma = []
+        for j in range(L + 2):
+            dp_, dm_ = a(j + 1) - a(j), a(j) - a(j - 1)
+            da = G[0][j] * (G[1][j] * dp_ + G[2][j] * dm_)
+            m3 = torch.minimum(torch.minimum(torch.abs(da), 2.0 * torch.abs(dm_)), 2.0 * torch.abs(dp_))
+            # Homme::min(a, b, c) selection with its tie rule
+            bc = torch.where(2.0 * torch.abs(dm_) < 2.0 * torch.abs(dp_), 2.0 * torch.abs(dm_), 2.0 * torch.abs(dp_))
+            m3 = torch.where(torch.abs(da) < bc, torch.abs(da), bc)
+            dma.append(torch.where(dp_ * dm_ > 0.0, m3 * torch.sign(da), torch.zeros_like(da)))
+        ai = []
+        for j in range(L + 1):
+            d1 = a(j + 1) - a(j)
+            ai.append(a(j) + G[3][j] * d1 + G[4][j] * (G[5][j] * (G[6][j] - G[7][j]) * d1 - G[8][j] * dma[j + 1]
+                                                      + G[9][j] * dma[j]))
+        c0, c1, c2 = [], [], []
+        for j in range(1, L + 1):
+            al, ar, aa = ai[j - 1], ai[j], a(j)
+            flat = (ar - aa) * (aa - al) <= 0.0
+            al = torch.where(flat, aa, al)
+            ar = torch.where(flat, aa, ar)
+            c2a = (ar - al) * (aa - (al + ar) / 2.0) > (ar - al) * (ar - al) / 6.0
+            al = torch.where(c2a, 3.0 * aa - 2.0 * ar, al)
+            c3a = (ar - al) * (aa - (al + ar) / 2.0) < -(ar - al) * (ar - al) / 6.0
+            ar = torch.where(c3a, 3.0 * aa - 2.0 * al, ar)
+            x0, x1, x2 = 1.5 * aa - (al + ar) / 4.0, ar - al, 3.0 * (-2.0 * aa + (al + ar))
+            if self.remap_alg == 2 and (j <= 2 or j >= L - 1):
+                x0, x1, x2 = aa, torch.zeros_like(aa), torch.zeros_like(aa)
+            c0.append(x0)
+            c1.append(x1)
+            c2.append(x2)
+        C0, C1, C2 = torch.stack(c0, 1), torch.stack(c1, 1), torch.stack(c2, 1)
+        mass = torch.cat([z, torch.cumsum(var, 1)], 1)              # mass(kk+1) = sum_{j<kk} var
+        outs = []
+        prev = None
+        for k in range(L):
+            kk = kid[:, k]
+            dxk = dx[rows, kk]
+            z2 = (pin[:, k + 1] - (pio[rows, kk] + pio[rows, kk + 1]) * 0.5) / dxk
+            x1, x2 = -0.5, z2
+            integ = (C0[rows, kk] * (x2 - x1) + C1[rows, kk] * (x2 * x2 - x1 * x1) / 2.0) + \
+                C2[rows, kk] * (x2 * x2 * x2 - x1 * x1 * x1) / 3.0
+            m = mass[rows, kk] + integ * dxk
+            outs.append(m if prev is None else m - prev)
+            prev = m
+        return torch.stack(outs, 1)
 
     def run(self, v, T, ps, nsteps):
         for _ in range(nsteps):
