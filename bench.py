@@ -231,6 +231,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-cost leg")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the adjoint-cost leg (rsplit 1 and 0); prints its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -240,6 +242,11 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.adjoint_only:
+        a1 = run_adjoint(1)
+        a1["rsplit0"] = run_adjoint(0)
+        print(json.dumps({"adjoint": a1}))
         return
     prob = make_problem(args)
     sec, prof, dry_drift, q_drift, st64 = run_gpu(args, prob, "fp64")

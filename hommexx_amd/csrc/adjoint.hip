@@ -131,15 +131,20 @@ __global__ __launch_bounds__(256) void adj_caar_scan_kernel(AdjCaarArgs a) {
   const size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= (size_t)a.nelem * 16) return;
   const int ie = (int)(col >> 4), pt = (int)(col & 15), L = a.nlev;
+  const size_t o = (size_t)ie * L * 16 + pt;
+  const double *__restrict__ dpc = a.dp + o;
+  const double *__restrict__ dvc = a.div + o;
+  double *__restrict__ pc = a.p + o;
+  double *__restrict__ cc = a.c + o;
   double p = a.p_top, dprev = 0.0, c = 0.0;
+#pragma unroll 8
   for (int k = 0; k < L; ++k) {
-    const size_t i = sidx(L, ie, k, pt);
-    const double dp = a.dp[i];
+    const double dp = dpc[(size_t)k * 16];
     p = p + 0.5 * (dprev + dp);
     dprev = dp;
-    a.p[i] = p;
-    a.c[i] = c;
-    c += a.div[i];
+    pc[(size_t)k * 16] = p;
+    cc[(size_t)k * 16] = c;
+    c += dvc[(size_t)k * 16];
   }
   a.stot[col] = c;
 }
@@ -240,27 +245,40 @@ __global__ __launch_bounds__(256) void adj_caar_b2_kernel(AdjCaarArgs a) {
   const size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= (size_t)a.nelem * 16) return;
   const int ie = (int)(col >> 4), pt = (int)(col & 15), L = a.nlev;
+  const size_t o = (size_t)ie * L * 16 + pt;
+  const double *__restrict__ phib = a.phib + o;
+  const double *__restrict__ pc = a.p + o;
+  const double *__restrict__ tc = a.t + o;
+  const double *__restrict__ dpc = a.dp + o;
+  const double *__restrict__ etab = a.etab + o;
+  const double *__restrict__ ombp = a.ombp + o;
+  double *__restrict__ lt = a.lt + o;
+  double *__restrict__ ldp = a.ldp + o;
+  double *__restrict__ pb = a.pb + o;
+  double *__restrict__ divb = a.divb + o;
   double phisum = 0.0, lstot = 0.0;
+#pragma unroll 8
   for (int k = 0; k < L; ++k) {
-    const size_t i = sidx(L, ie, k, pt);
-    const double phib = a.phib[i];
-    const double lr = phib + 2.0 * phisum;
-    phisum += phib;
-    const double P = a.p[i], T = a.t[i], dp = a.dp[i];
-    a.lt[i] += RGAS * dp * 0.5 / P * lr;
-    a.ldp[i] += RGAS * T * 0.5 / P * lr;
-    a.pb[i] -= RGAS * T * dp * 0.5 / (P * P) * lr;
-    lstot += a.hybi[k] * a.etab[i];
+    const size_t i = (size_t)k * 16;
+    const double ph = phib[i];
+    const double lr = ph + 2.0 * phisum;
+    phisum += ph;
+    const double P = pc[i], T = tc[i], dp = dpc[i];
+    lt[i] += RGAS * dp * 0.5 / P * lr;
+    ldp[i] += RGAS * T * 0.5 / P * lr;
+    pb[i] -= RGAS * T * dp * 0.5 / (P * P) * lr;
+    lstot += a.hybi[k] * etab[i];
   }
   double spb = 0.0, scb = 0.0;
+#pragma unroll 8
   for (int k = L - 1; k >= 0; --k) {
-    const size_t i = sidx(L, ie, k, pt);
-    const double pb = a.pb[i];
-    a.ldp[i] += 0.5 * pb + spb;
-    spb += pb;
-    const double ombp = a.ombp[i];
-    a.divb[i] += scb + lstot - 0.5 * ombp;
-    scb += -a.etab[i] - ombp;
+    const size_t i = (size_t)k * 16;
+    const double b = pb[i];
+    ldp[i] += 0.5 * b + spb;
+    spb += b;
+    const double om = ombp[i];
+    divb[i] += scb + lstot - 0.5 * om;
+    scb += -etab[i] - om;
   }
 }
 
@@ -555,10 +573,10 @@ __global__ __launch_bounds__(256) void adj_remap_field_kernel(AdjRemapArgs a) {
   auto A = [&](int j) -> double & { return F(RF_AA, j + 1); };   // a(j), j = -1..L+2
   auto LA = [&](int j) -> double & { return F(RF_LA, j + 1); };
   const int W = L + 4;
+  // sums over the targets of a source cell: cells without targets stay zero
   for (int i = 0; i < W; ++i) {
-    F(RF_LA, i) = 0.0; F(RF_LDMA, i) = 0.0; F(RF_LAI, i) = 0.0; F(RF_LC0, i) = 0.0; F(RF_LC1, i) = 0.0;
-    F(RF_LC2, i) = 0.0; F(RF_LMASS, i) = 0.0; F(RF_LDX, i) = 0.0; F(RF_LD, i) = 0.0; F(RF_LPIO, i) = 0.0;
-    F(RF_LPIN, i) = 0.0; F(RF_LTG, i) = 0.0;
+    F(RF_LC0, i) = 0.0; F(RF_LC1, i) = 0.0; F(RF_LC2, i) = 0.0; F(RF_LMASS, i) = 0.0; F(RF_LDX, i) = 0.0;
+    F(RF_LPIO, i) = 0.0; F(RF_LPIN, i) = 0.0;
   }
   // ---------------------------------------------------------- forward
   for (int k = 0; k < L; ++k) {
@@ -602,11 +620,33 @@ __global__ __launch_bounds__(256) void adj_remap_field_kernel(AdjRemapArgs a) {
     bnd = a.alg == 2 && (j <= 2 || j >= L - 1);
     if (bnd) { c0 = aa; c1 = 0.0; c2 = 0.0; }
   };
+  // The reverse passes accumulate in registers over a sliding window of
+  // levels and write each scratch entry once (or add to it once), so no
+  // entry is read back while its own update is still in flight.
   // ------------------------------------------ reverse: targets (ascending)
+  // kid is non-decreasing in k: the targets of one source cell are
+  // consecutive, and the cell's sums are written when the cell changes
   double prefix = 0.0, mprev = 0.0;
-  int jj = 0;
+  int jj = 0, cur = -1;
+  double s_lc0 = 0.0, s_lc1 = 0.0, s_lc2 = 0.0, s_lm = 0.0, s_ldx = 0.0, s_plo = 0.0, s_phi = 0.0;
+  auto flush_cell = [&]() {
+    if (cur < 0) return;
+    F(RF_LC0, cur) = s_lc0;
+    F(RF_LC1, cur) = s_lc1;
+    F(RF_LC2, cur) = s_lc2;
+    F(RF_LMASS, cur) = s_lm;
+    F(RF_LDX, cur) = s_ldx;
+    F(RF_LPIO, cur) += s_plo;
+    F(RF_LPIO, cur + 1) += s_phi;
+  };
+  double lcur = lam[fidx(0)] / c.g(RG_TG, 0);
   for (int k = 0; k < L; ++k) {
     const int kk = (int)c.g(RG_KID, k);
+    if (kk != cur) {
+      flush_cell();
+      cur = kk;
+      s_lc0 = s_lc1 = s_lc2 = s_lm = s_ldx = s_plo = s_phi = 0.0;
+    }
     while (jj < kk) { prefix += fin[fidx(jj)] * src(jj); ++jj; }
     const double z2 = c.g(RG_Z2, k), dxk = src(kk), tg = c.g(RG_TG, k);
     double c0, c1, c2; bool fl, f2, f3, bd;
@@ -616,21 +656,22 @@ __global__ __launch_bounds__(256) void adj_remap_field_kernel(AdjRemapArgs a) {
     const double out = k > 0 ? m - mprev : m;
     mprev = m;
     const double lf = lam[fidx(k)];
-    F(RF_LTG, k) += -lf * (out / tg) / tg;
+    F(RF_LTG, k) = -lf * (out / tg) / tg;
     const double lnext = k + 1 < L ? lam[fidx(k + 1)] / c.g(RG_TG, k + 1) : 0.0;
-    const double lm = lf / tg - lnext;
-    F(RF_LMASS, kk) += lm;
+    const double lm = lcur - lnext;
+    lcur = lnext;
     const double li = lm * dxk;
-    F(RF_LDX, kk) += lm * integ;
-    F(RF_LC0, kk) += li * (z2 + 0.5);
-    F(RF_LC1, kk) += li * (z2 * z2 - 0.25) / 2.0;
-    F(RF_LC2, kk) += li * (z2 * z2 * z2 + 0.125) / 3.0;
     const double lz2 = li * (c0 + c1 * z2 + c2 * z2 * z2);
-    F(RF_LPIN, k + 1) += lz2 / dxk;
-    F(RF_LPIO, kk) -= 0.5 * lz2 / dxk;
-    F(RF_LPIO, kk + 1) -= 0.5 * lz2 / dxk;
-    F(RF_LDX, kk) -= lz2 * z2 / dxk;
+    s_lm += lm;
+    s_ldx += lm * integ - lz2 * z2 / dxk;
+    s_lc0 += li * (z2 + 0.5);
+    s_lc1 += li * (z2 * z2 - 0.25) / 2.0;
+    s_lc2 += li * (z2 * z2 * z2 + 0.125) / 3.0;
+    s_plo -= 0.5 * lz2 / dxk;
+    s_phi -= 0.5 * lz2 / dxk;
+    F(RF_LPIN, k + 1) = lz2 / dxk;
   }
+  flush_cell();
   // mass prefix -> lambda of var: LMASS(j) becomes sum_{i > j} LMASS(i)
   {
     double s = F(RF_LMASS, L);
@@ -640,82 +681,103 @@ __global__ __launch_bounds__(256) void adj_remap_field_kernel(AdjRemapArgs a) {
       s += o;
     }
   }
-  // cells (descending): coefficients -> a(j), ai(j-1), ai(j)
+  // cells (descending): coefficients -> a(j), ai(j-1), ai(j); first writer of
+  // LAI (all) and LA(1..L)
+  LA(-1) = 0.0; LA(0) = 0.0; LA(L + 1) = 0.0; LA(L + 2) = 0.0;
+  double lai_carry = 0.0;  // ai(j) as the left edge of cell j+1
   for (int j = L; j >= 1; --j) {
     const int cidx = j - 1;
     double c0, c1, c2; bool flat, f2, f3, bnd;
     parabola(j, c0, c1, c2, flat, f2, f3, bnd);
     const double l0 = F(RF_LC0, cidx), l1 = F(RF_LC1, cidx), l2 = F(RF_LC2, cidx);
-    if (bnd) { LA(j) += l0; continue; }
-    double laa = 1.5 * l0 - 6.0 * l2;
-    double lal = -l0 / 4.0 - l1 + 3.0 * l2;
-    double lar = -l0 / 4.0 + l1 + 3.0 * l2;
-    if (f3) { laa += 3.0 * lar; lal += -2.0 * lar; lar = 0.0; }
-    if (f2) { laa += 3.0 * lal; lar += -2.0 * lal; lal = 0.0; }
-    if (flat) { laa += lal + lar; lal = 0.0; lar = 0.0; }
-    F(RF_LAI, j - 1) += lal;
-    F(RF_LAI, j) += lar;
-    LA(j) += laa;
+    double laa, lal, lar;
+    if (bnd) {
+      laa = l0; lal = 0.0; lar = 0.0;
+    } else {
+      laa = 1.5 * l0 - 6.0 * l2;
+      lal = -l0 / 4.0 - l1 + 3.0 * l2;
+      lar = -l0 / 4.0 + l1 + 3.0 * l2;
+      if (f3) { laa += 3.0 * lar; lal += -2.0 * lar; lar = 0.0; }
+      if (f2) { laa += 3.0 * lal; lar += -2.0 * lal; lal = 0.0; }
+      if (flat) { laa += lal + lar; lal = 0.0; lar = 0.0; }
+    }
+    F(RF_LAI, j) = lai_carry + lar;
+    lai_carry = lal;
+    LA(j) = laa;
   }
-  // interfaces ai(j), j = L..0
-  for (int j = L; j >= 0; --j) {
-    for (int m = 0; m < 4; ++m) X[m] = DX(j + m);
-    double gg[10], dd[10][4];
-    for (int i = 3; i < 10; ++i) gg[i] = ppm_g(i, X, dd[i]);
-    const double li = F(RF_LAI, j);
-    const double d1 = A(j + 1) - A(j);
-    const double g56 = gg[5] * (gg[6] - gg[7]);
-    const double dmaj = F(RF_DMA, j), dmaj1 = F(RF_DMA, j + 1);
-    LA(j) += li;
-    const double ld1 = li * (gg[3] + gg[4] * g56);
-    LA(j + 1) += ld1;
-    LA(j) -= ld1;
-    F(RF_LDMA, j + 1) -= li * gg[4] * gg[8];
-    F(RF_LDMA, j) += li * gg[4] * gg[9];
-    double lg[10] = {0};
-    lg[3] = li * d1;
-    lg[4] = li * (g56 * d1 - gg[8] * dmaj1 + gg[9] * dmaj);
-    const double lg56 = li * gg[4] * d1;
-    lg[5] = lg56 * (gg[6] - gg[7]);
-    lg[6] = lg56 * gg[5];
-    lg[7] = -lg56 * gg[5];
-    lg[8] = -li * gg[4] * dmaj1;
-    lg[9] = li * gg[4] * dmaj;
-    for (int i = 3; i < 10; ++i)
-      for (int m = 0; m < 4; ++m) F(RF_LD, j + m) += lg[i] * dd[i][m];
+  F(RF_LAI, 0) = lai_carry;
+  // interfaces ai(j), j = L..0; first writer of LDMA and LD
+  {
+    double la_carry = 0.0, ldma_carry = 0.0, w[4] = {0.0, 0.0, 0.0, 0.0};  // w[m]: LD(j+m)
+    for (int j = L; j >= 0; --j) {
+      if (j + 4 <= L + 3) F(RF_LD, j + 4) = w[3];
+      w[3] = w[2]; w[2] = w[1]; w[1] = w[0]; w[0] = 0.0;
+      for (int m = 0; m < 4; ++m) X[m] = DX(j + m);
+      double gg[10], dd[10][4];
+      for (int i = 3; i < 10; ++i) gg[i] = ppm_g(i, X, dd[i]);
+      const double li = F(RF_LAI, j);
+      const double d1 = A(j + 1) - A(j);
+      const double g56 = gg[5] * (gg[6] - gg[7]);
+      const double dmaj = F(RF_DMA, j), dmaj1 = F(RF_DMA, j + 1);
+      const double ld1 = li * (gg[3] + gg[4] * g56);
+      LA(j + 1) += la_carry + ld1;
+      la_carry = li - ld1;
+      F(RF_LDMA, j + 1) = ldma_carry - li * gg[4] * gg[8];
+      ldma_carry = li * gg[4] * gg[9];
+      double lg[10] = {0};
+      lg[3] = li * d1;
+      lg[4] = li * (g56 * d1 - gg[8] * dmaj1 + gg[9] * dmaj);
+      const double lg56 = li * gg[4] * d1;
+      lg[5] = lg56 * (gg[6] - gg[7]);
+      lg[6] = lg56 * gg[5];
+      lg[7] = -lg56 * gg[5];
+      lg[8] = -li * gg[4] * dmaj1;
+      lg[9] = li * gg[4] * dmaj;
+      for (int i = 3; i < 10; ++i)
+        for (int m = 0; m < 4; ++m) w[m] += lg[i] * dd[i][m];
+    }
+    LA(0) += la_carry;
+    F(RF_LDMA, 0) = ldma_carry;
+    for (int m = 0; m < 4; ++m) F(RF_LD, m) = w[m];
   }
   // dma(j), j = L+1..0
-  for (int j = L + 1; j >= 0; --j) {
-    for (int m = 0; m < 4; ++m) X[m] = DX(j + m < L + 3 ? j + m : L + 3);
-    double d0[4], d1[4], d2[4];
-    const double g0 = ppm_g(0, X, d0), g1 = ppm_g(1, X, d1), g2 = ppm_g(2, X, d2);
-    const double dp_ = A(j + 1) - A(j), dm_ = A(j) - A(j - 1);
-    if (!(dp_ * dm_ > 0.0)) continue;
-    const double da = g0 * (g1 * dp_ + g2 * dm_);
-    const bool bsel = 2.0 * fabs(dm_) < 2.0 * fabs(dp_);
-    const double bc = bsel ? 2.0 * fabs(dm_) : 2.0 * fabs(dp_);
-    const bool asel = fabs(da) < bc;
-    const double sda = da > 0.0 ? 1.0 : da < 0.0 ? -1.0 : 0.0;
-    const double lm3 = F(RF_LDMA, j) * sda;
-    double lda = 0.0, ldp = 0.0, ldm = 0.0;
-    if (asel) {
-      lda = lm3 * sda;
-    } else if (bsel) {
-      ldm = 2.0 * (dm_ > 0.0 ? 1.0 : dm_ < 0.0 ? -1.0 : 0.0) * lm3;
-    } else {
-      ldp = 2.0 * (dp_ > 0.0 ? 1.0 : dp_ < 0.0 ? -1.0 : 0.0) * lm3;
+  {
+    double c1 = 0.0, c2 = 0.0, w[3] = {0.0, 0.0, 0.0};  // c1: LA(j+1) pending, c2: LA(j); w[m]: LD(j+m)
+    for (int j = L + 1; j >= 0; --j) {
+      if (j + 3 <= L + 3) F(RF_LD, j + 3) += w[2];
+      w[2] = w[1]; w[1] = w[0]; w[0] = 0.0;
+      double ldp = 0.0, ldm = 0.0;
+      const double dp_ = A(j + 1) - A(j), dm_ = A(j) - A(j - 1);
+      if (dp_ * dm_ > 0.0) {
+        for (int m = 0; m < 4; ++m) X[m] = DX(j + m < L + 3 ? j + m : L + 3);
+        double d0[4], d1[4], d2[4];
+        const double g0 = ppm_g(0, X, d0), g1 = ppm_g(1, X, d1), g2 = ppm_g(2, X, d2);
+        const double da = g0 * (g1 * dp_ + g2 * dm_);
+        const bool bsel = 2.0 * fabs(dm_) < 2.0 * fabs(dp_);
+        const double bc = bsel ? 2.0 * fabs(dm_) : 2.0 * fabs(dp_);
+        const bool asel = fabs(da) < bc;
+        const double sda = da > 0.0 ? 1.0 : da < 0.0 ? -1.0 : 0.0;
+        const double lm3 = F(RF_LDMA, j) * sda;
+        double lda = 0.0;
+        if (asel) {
+          lda = lm3 * sda;
+        } else if (bsel) {
+          ldm = 2.0 * (dm_ > 0.0 ? 1.0 : dm_ < 0.0 ? -1.0 : 0.0) * lm3;
+        } else {
+          ldp = 2.0 * (dp_ > 0.0 ? 1.0 : dp_ < 0.0 ? -1.0 : 0.0) * lm3;
+        }
+        const double lg0 = lda * (g1 * dp_ + g2 * dm_), lg1 = lda * g0 * dp_, lg2 = lda * g0 * dm_;
+        ldp += lda * g0 * g1;
+        ldm += lda * g0 * g2;
+        for (int m = 0; m < 3; ++m) w[m] += lg0 * d0[m] + lg1 * d1[m] + lg2 * d2[m];
+      }
+      LA(j + 1) += c1 + ldp;
+      c1 = c2 + (ldm - ldp);
+      c2 = -ldm;
     }
-    const double lg0 = lda * (g1 * dp_ + g2 * dm_), lg1 = lda * g0 * dp_, lg2 = lda * g0 * dm_;
-    ldp += lda * g0 * g1;
-    ldm += lda * g0 * g2;
-    LA(j + 1) += ldp;
-    LA(j) -= ldp;
-    LA(j) += ldm;
-    LA(j - 1) -= ldm;
-    for (int m = 0; m < 3; ++m) {
-      const int i = j + m < L + 3 ? j + m : L + 3;
-      F(RF_LD, i) += lg0 * d0[m] + lg1 * d1[m] + lg2 * d2[m];
-    }
+    LA(0) += c1;
+    LA(-1) += c2;
+    for (int m = 0; m < 3; ++m) F(RF_LD, m) += w[m];
   }
   // ghosts of a fold back; a = var/dx, var = f*dx
   for (int k = 0; k < L; ++k) {
