@@ -166,7 +166,7 @@ __device__ __forceinline__ void limit9_slab(double minp, double maxp, double &x,
 // (min/max with the previous bounds, no neighbour exchange) runs first.
 //
 // GATHER (stage 2 only): the stage-1 exchange is done here instead of by a
-// separate kernel.  A block is one element x 8 levels (128 lanes); lanes
+// separate kernel.  A block is one element x AGL levels (16*AGL lanes); lanes
 // (pt, kk) gather the stage-1 pre-exchange values (levels fastest) of point
 // pt at level k0+kk with the reference's accumulation order, an LDS tile
 // hands them to the slab lanes (kk, pt), and the assembled stage-1 Qdp is
@@ -193,8 +193,8 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
     pt = threadIdx.x & 15;
     k = k0 + kk;
     son = k < L;
-    gpt = threadIdx.x >> 3;
-    gkk = threadIdx.x & 7;
+    gpt = threadIdx.x / AGL;
+    gkk = threadIdx.x % AGL;
     gk = k0 + gkk;
     gon = gk < L;
     const int *src = a.src + ((size_t)ie * 16 + gpt) * 3;

@@ -89,8 +89,16 @@ __device__ __forceinline__ void ppm_parabola(double al, double ar, double a, dou
     al = a;
     ar = a;
   }
-  if ((ar - al) * (a - (al + ar) / 2.0) > (ar - al) * (ar - al) / 6.0) al = 3.0 * a - 2.0 * ar;
-  if ((ar - al) * (a - (al + ar) / 2.0) < -(ar - al) * (ar - al) / 6.0) ar = 3.0 * a - 2.0 * al;
+  // the two monotonicity tests share (ar - al) and (ar - al)^2 / 6 unless
+  // the first one moves al (same operations and rounding as evaluating each
+  // test in full)
+  double d = ar - al, q = d * d / 6.0;
+  if (d * (a - (al + ar) / 2.0) > q) {
+    al = 3.0 * a - 2.0 * ar;
+    d = ar - al;
+    q = d * d / 6.0;
+  }
+  if (d * (a - (al + ar) / 2.0) < -q) ar = 3.0 * a - 2.0 * al;
   c0 = 1.5 * a - (al + ar) / 4.0;
   c1 = ar - al;
   c2 = 3.0 * (-2.0 * a + (al + ar));
