@@ -28,13 +28,17 @@ __device__ __forceinline__ int lo32(double x) { return __double2loint(x); }
 __device__ __forceinline__ int hi32(double x) { return __double2hiint(x); }
 __device__ __forceinline__ double mk64(int hi, int lo) { return __hiloint2double(hi, lo); }
 
+// DPP moves without an "old" operand: with bound_ctrl set, a lane whose
+// source is outside its row reads 0, so the destination needs no zeroing
+// move first (quad_perm sources are always inside the quad).
+template <int CTRL>
+__device__ __forceinline__ int dpp_mov(int x) { return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, true); }
+
 // DPP quad_perm broadcast of lane K of each quad (same igp row, jgp = K).
 template <int K>
 __device__ __forceinline__ double row_get(double x) {
   constexpr int ctrl = K * 0x55;  // quad_perm:[K,K,K,K]
-  const int l = __builtin_amdgcn_update_dpp(0, lo32(x), ctrl, 0xF, 0xF, false);
-  const int h = __builtin_amdgcn_update_dpp(0, hi32(x), ctrl, 0xF, 0xF, false);
-  return mk64(h, l);
+  return mk64(dpp_mov<ctrl>(hi32(x)), dpp_mov<ctrl>(lo32(x)));
 }
 
 // ds_swizzle bit-mask mode within 32 lanes: src = (lane & 0x13) | (K << 2):
@@ -56,11 +60,7 @@ __device__ __forceinline__ double row_last(double x) {
 }
 
 // row_shr:1 within 16-lane rows, lane 0 of each row receives 0.0
-__device__ __forceinline__ double shr1(double x) {
-  const int l = __builtin_amdgcn_update_dpp(0, lo32(x), 0x111, 0xF, 0xF, false);
-  const int h = __builtin_amdgcn_update_dpp(0, hi32(x), 0x111, 0xF, 0xF, false);
-  return mk64(h, l);
-}
+__device__ __forceinline__ double shr1(double x) { return mk64(dpp_mov<0x111>(hi32(x)), dpp_mov<0x111>(lo32(x))); }
 
 // Serial (left-to-right over points 0..15) sum, result in every lane of the slab.
 __device__ __forceinline__ double slab_sum_serial(double v) {
@@ -168,7 +168,7 @@ namespace hx {
 
 template <int K>
 __device__ __forceinline__ float row_getf(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), K * 0x55, 0xF, 0xF, false));
+  return __int_as_float(dpp_mov<K * 0x55>(__float_as_int(x)));
 }
 template <int K>
 __device__ __forceinline__ float col_getf(float x) {
