@@ -238,7 +238,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the preqx step here is built and measured on a single MI355X; "
-              "multi-GPU runs are not part of this build (see DESIGN.md for how it would shard)", file=sys.stderr)
+              "multi-GPU runs are not part of this build (the step would shard by elements, with one halo "
+              "exchange per DSS)", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
