@@ -1235,11 +1235,12 @@ void f90_push_forcing_to_cxx(double *FM, double *FT, double *FQ, double *Qdp) {
 
 // ============================================================ extensions
 // Dimensions the reference fixes at compile time (PLEV, QSIZE_D); call before
-// init_elements_2d_c.  Defaults: nlev = 72, qsize_d = qsize.
+// init_elements_2d_c.  Defaults: nlev = 72, qsize_d = qsize.  nlev is at most
+// REMAP_MAX_NLEV: the remap keeps each element's target table in LDS.
 int hommexx_hip_configure(int nlev, int qsize_d) {
   Ctx &c = C();
   if (c.elems_set) return 1;
-  if (nlev < 3 || nlev > 4096) return 2;
+  if (nlev < 3 || nlev > REMAP_MAX_NLEV) return 2;
   c.nlev_cfg = nlev;
   c.qsize_d_cfg = qsize_d;
   return 0;

@@ -168,6 +168,9 @@ void launch_state_forcing(hipStream_t st, int nelem, int nlev, double dt, const 
                           double *t, double *v);
 void launch_tracer_forcing(hipStream_t st, int nelem, int nlev, int qsize, bool moist, double dt, const double *fq,
                            double *qdp, double *Q, const double *hyai, const double *hybi, double ps0, double *ps);
+// largest nlev of the remap kernel: an element's target table (10 B per
+// level and column) and a grid chunk (20 KB) fit in 64 KB of LDS
+constexpr int REMAP_MAX_NLEV = 256;
 void launch_remap(hipStream_t st, const RemapArgs &a);
 void launch_zero_int(hipStream_t st, int *p);
 

@@ -104,234 +104,272 @@ __device__ __forceinline__ void ppm_parabola(double al, double ar, double a, dou
   c2 = 3.0 * (-2.0 * a + (al + ar));
 }
 
-// One lane = one GLL column and a batch of RFB fields (field 0,1: v
-// components, 2: T, >= 3: tracer field-3).  A wave is 64 columns (four
-// elements) of one field batch, so the field kinds are wave-uniform.  The
-// grid factors, the target search (kid, z2) and the control flow of the sweep
-// depend on the column only and are shared by the batch.
+// One block = one element (16 GLL columns) and up to 16*RFB fields; lane
+// (row, pt) sweeps column pt for fields f0 + (16*blockIdx.y + row)*RFB + i
+// (field 0,1: v components, 2: T, >= 3: tracer field-3).  What depends on
+// the column only is computed once per block and shared through LDS:
+// * the target table: for each target level k the source cell kid(k) that
+//   holds its lower interface and the offset z2(k) in it (the reference's
+//   kid and z2 arrays, the same forward search), found by the 16 lanes of
+//   row 0 before the sweep;
+// * the grid factors of RCH sweep iterations at a time (the reference's
+//   ppmdx rows for dma(j+1) and ai(j), dx(j) and the incoming dx(j+3)),
+//   iteration j0 + row computed by row row.
+constexpr int RCH = 16;  // sweep iterations per grid chunk (rows 0..RCH-1 compute one each)
+constexpr int RG = 10;   // shared grid values per (iteration, column)
+
+size_t remap_lds_bytes(int nlev) { return (size_t)RCH * RG * 16 * 8 + (size_t)nlev * (16 * (8 + 2) + 2 * 8); }
+
 template <int RFB>  // fields per lane
-__global__ __launch_bounds__(256) void remap_kernel(RemapArgs a, int nfb) {
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int w = lb * 4 + (threadIdx.x >> 6);  // global wave
-  const int eg = w / nfb, fb = w % nfb;
-  const int ie = eg * 4 + ((threadIdx.x >> 4) & 3);
-  if (ie >= a.nelem) return;
-  const int pt = threadIdx.x & 15;
-  const int nf = 3 + a.qsize;  // in the field numbering below (0,1: v, 2: T, 3..: tracers)
-  const int f0 = 3 - a.nstates;  // rsplit == 0: tracers only
-  const size_t col = (size_t)ie * 16 + pt;
+__global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
+  extern __shared__ double rsm[];
   const int L = a.nlev;
+  double *gsh = rsm;                              // [RCH][RG][16]
+  double *zsh = rsm + RCH * RG * 16;              // z2  [L][16]
+  double *dab = zsh + (size_t)L * 16;             // hyai(k+1) - hyai(k), hybi(k+1) - hybi(k)  [L][2]
+  short *ksh = (short *)(dab + (size_t)L * 2);    // kid [L][16]
+  const int ie = xcd_block(blockIdx.x, gridDim.x);
+  const int row = threadIdx.x >> 4, pt = threadIdx.x & 15;
+  const int nf = 3 + a.qsize;    // in the field numbering above
+  const int f0 = 3 - a.nstates;  // rsplit == 0: tracers only
+  const int fb = f0 + (blockIdx.y * 16 + row) * RFB;
+  const bool active = fb < nf;
+  const size_t col = (size_t)ie * 16 + pt;
   const double *src = (a.nstates ? a.dp : a.src) + (size_t)ie * L * 16 + pt;
+  const double ps = a.ps[col];
+  // wait for that load on every path here (s_waitcnt vmcnt(0); expcnt and
+  // lgkmcnt left at their maxima: 0x0F70 on gfx9): only row 0 uses ps before
+  // the sweep, and without the wait the compiler's waits inside the sweep
+  // assume it may still be in flight and also wait for the prefetched loads
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  auto tgt_of = [&](int k) {
+    return (a.hyai[k + 1] - a.hyai[k]) * a.ps0 + (a.hybi[k + 1] - a.hybi[k]) * ps;
+  };
+  // dx of Fortran cell m = 1..L with the mirrored ghosts dx(0) = dx(1),
+  // dx(-1) = dx(2), dx(L+1) = dx(L), dx(L+2) = dx(L-1)
+  auto dxf = [&](int m) {
+    int c = m - 1;
+    c = c < 0 ? -c - 1 : (c >= L ? 2 * L - 1 - c : c);
+    return src[(size_t)c * 16];
+  };
+
+  // ------------------------------------------------------------ targets
+  for (int k = threadIdx.x; k < L; k += blockDim.x) {
+    dab[2 * k] = a.hyai[k + 1] - a.hyai[k];
+    dab[2 * k + 1] = a.hybi[k + 1] - a.hybi[k];
+  }
+  if (row == 0) {
+    double pio_L = 0.0;  // pio(L): serial sum of the source thicknesses
+    for (int c = 0; c < L; ++c) pio_L += src[(size_t)c * 16];
+    double pin_acc = 0.0, pio_si = 0.0, pio_m1 = 0.0, pio_m2 = 0.0;  // pio(si), pio(si-1), pio(si-2)
+    int si = 0;
+    auto advance_si = [&]() {
+      pio_m2 = pio_m1;
+      pio_m1 = pio_si;
+      pio_si = si < L ? pio_si + src[(size_t)si * 16] : pio_si + 1.0;  // pio(L+1) = pio(L) + 1
+      ++si;
+    };
+    for (int k = 0; k < L; ++k) {
+      double pin_k1;
+      if (k + 1 < L) {
+        pin_acc += tgt_of(k);
+        pin_k1 = pin_acc;
+      } else {
+        pin_k1 = pio_L;  // pin(L) = pio(L)
+      }
+      while (si < k) advance_si();  // the search for target k starts at interface k
+      while (si <= L && pio_si <= pin_k1) advance_si();
+      int kk = si;
+      double plo = pio_m1, phi = pio_si;  // pio(kk-1), pio(kk)
+      if (kk == L + 1) {
+        kk = L;
+        plo = pio_m2;
+        phi = pio_m1;
+      }
+      const int kid = kk - 1;
+      ksh[k * 16 + pt] = (short)kid;
+      zsh[k * 16 + pt] = kid >= 0 ? (pin_k1 - (plo + phi) * 0.5) / src[(size_t)kid * 16] : 0.0;
+    }
+  }
+
+  // ------------------------------------------------------------ fields
+  // (pointers of absent fields stay valid device addresses, never null, so
+  // every access compiles to a global, not a flat, instruction)
   double *var[RFB];
   double *Qout[RFB];
   size_t vs[RFB];
-  bool on[RFB], state[RFB];
+  bool on[RFB], state[RFB], qout[RFB];
 #pragma unroll
   for (int i = 0; i < RFB; ++i) {
-    const int f = f0 + fb * RFB + i;
+    const int f = fb + i;
     on[i] = f < nf;
     state[i] = f < 3;
+    qout[i] = f >= 3 && f < nf && a.Q != nullptr;
     vs[i] = f < 2 ? 32 : 16;
-    Qout[i] = nullptr;
+    var[i] = a.t;
+    Qout[i] = a.t;
     if (f < 2) var[i] = a.v + (size_t)ie * L * 32 + f * 16 + pt;
     else if (f == 2) var[i] = a.t + (size_t)ie * L * 16 + pt;
     else if (f < nf) {
       var[i] = a.qdp + (((size_t)ie * a.qsize + (f - 3)) * L) * 16 + pt;
-      if (a.Q) Qout[i] = a.Q + (((size_t)ie * a.qsize + (f - 3)) * L) * 16 + pt;
-    } else var[i] = nullptr;
+      if (qout[i]) Qout[i] = a.Q + (((size_t)ie * a.qsize + (f - 3)) * L) * 16 + pt;
+    }
   }
-  const double ps = a.ps[col];
   const int alg = a.remap_alg;
-  auto tgt_of = [&](int k) {
-    return (a.hyai[k + 1] - a.hyai[k]) * a.ps0 + (a.hybi[k + 1] - a.hybi[k]) * ps;
-  };
-  // pio(L): serial sum of the source thicknesses
-  double pio_L = 0.0;
-  for (int c = 0; c < L; ++c) pio_L += src[(size_t)c * 16];
-
-  // ------------------------------------------------------------ window
-  // Fortran-style cell indices j = 1..L, mirrored (or constant) ghosts.
-  // (Xm1, X0, X1, X2) = dx(j-1..j+2); per field (Am1..A2) = a(j-1..j+2),
+  // Fortran-style cell indices j = 1..L.  Per field (A0..A2) = a(j..j+2),
   // V0..V2 = var(j..j+2)
-  const double s0 = src[0], s1 = src[16];
-  double Xm1 = s1, X0 = s0, X1 = s0, X2 = s1;
-  double Am1[RFB], A0[RFB], A1[RFB], A2[RFB], V0[RFB], V1[RFB], V2[RFB];
+  double A0[RFB], A1[RFB], A2[RFB], V0[RFB], V1[RFB], V2[RFB];
   double dma_j[RFB], ai_prev[RFB], var_prefix[RFB], mass_prev[RFB];
   double cc0[RFB], cc1[RFB], cc2[RFB];
-  double g0, g1, g2;  // dma factors of the current window
-  {
-    g0 = X0 / (Xm1 + X0 + X1);
-    g1 = (2.0 * Xm1 + X0) / (X0 + X1);
-    g2 = (X0 + 2.0 * X1) / (Xm1 + X0);
-  }
-#pragma unroll
-  for (int i = 0; i < RFB; ++i) {
-    if (!on[i]) continue;
-    const double x0 = var[i][0], x1 = var[i][vs[i]];
-    const double v0 = state[i] ? x0 * s0 : x0, v1 = state[i] ? x1 * s1 : x1;
-    const double a1 = v0 / s0, a2 = v1 / s1;
-    Am1[i] = alg == 3 ? a1 : a2;
-    A0[i] = a1;
-    A1[i] = a1;
-    A2[i] = a2;
-    V0[i] = 0.0;
-    V1[i] = v0;
-    V2[i] = v1;
-    dma_j[i] = ppm_dma_g(Am1[i], A0[i], A1[i], g0, g1, g2);  // dma(0)
-    ai_prev[i] = 0.0;
-    var_prefix[i] = 0.0;
-    mass_prev[i] = 0.0;
-  }
-
-  // ------------------------------------------------------------ targets
-  int k = 0;     // next target level
-  int kid = -2;  // kid of target k (-2: not yet computed)
-  double z2 = 0.0, pin_acc = 0.0;
-  int si = 0;
-  double pio_si = 0.0, pio_m1 = 0.0, pio_m2 = 0.0;  // pio(si), pio(si-1), pio(si-2)
-
-  auto advance_si = [&]() {
-    pio_m2 = pio_m1;
-    pio_m1 = pio_si;
-    pio_si = si < L ? pio_si + src[(size_t)si * 16] : pio_si + 1.0;  // pio(L+1) = pio(L) + 1
-    ++si;
-  };
-  auto compute_kid = [&]() {
-    double pin_k1;
-    if (k + 1 < L) {
-      pin_acc += tgt_of(k);
-      pin_k1 = pin_acc;
-    } else {
-      pin_k1 = pio_L;  // pin(L) = pio(L)
-    }
-    while (si < k) advance_si();  // the search for target k starts at interface k
-    while (si <= L && pio_si <= pin_k1) advance_si();
-    int kk = si;
-    double plo = pio_m1, phi = pio_si;  // pio(kk-1), pio(kk)
-    if (kk == L + 1) {
-      kk = L;
-      plo = pio_m2;
-      phi = pio_m1;
-    }
-    kid = kk - 1;
-    z2 = kid >= 0 ? (pin_k1 - (plo + phi) * 0.5) / src[(size_t)kid * 16] : 0.0;
-  };
-  // finish target k, whose upper interface lies in the current cell
-  auto finish = [&](double dx) {
-    const double tk = tgt_of(k);
-#pragma unroll
-    for (int i = 0; i < RFB; ++i) {
-      if (!on[i]) continue;
-      const double integral = integrate_parabola(cc2[i], cc1[i], cc0[i], -0.5, z2);
-      const double mass = var_prefix[i] + integral * dx;
-      const double out = k > 0 ? mass - mass_prev[i] : mass;
-      mass_prev[i] = mass;
-      if (state[i]) {
-        var[i][(size_t)k * vs[i]] = out / tk;
-      } else {
-        var[i][(size_t)k * vs[i]] = out;
-        if (Qout[i]) Qout[i][(size_t)k * 16] = out / tk;
-      }
-    }
-    ++k;
-    kid = -2;
-  };
-
-  // ------------------------------------------------------------ sweep
-  // cell j+3 enters the window at iteration j; its dx and field values are
-  // loaded two iterations ahead so the load latency hides behind sweep steps
-  double px = 0.0, pv[RFB], qx = 0.0, qv[RFB];
-  auto fetch = [&](int m, double &x, double *v) {
+  double pv[RFB], qv[RFB];  // field values of cells j+3, j+4 (loaded ahead)
+  auto fetch = [&](int m, double *v) {
     if (m <= L) {
-      x = src[(size_t)(m - 1) * 16];
 #pragma unroll
       for (int i = 0; i < RFB; ++i)
         if (on[i]) v[i] = var[i][(size_t)(m - 1) * vs[i]];
     }
   };
-  fetch(3, px, pv);
-  fetch(4, qx, qv);
-  for (int j = 0; j <= L; ++j) {
-    double nx = 0.0, nv[RFB];
-    fetch(j + 5, nx, nv);
-    // dma(j+1) factors: cells j..j+2; ai(j) factors: cells j-1..j+2
-    const double h0 = X1 / (X0 + X1 + X2);
-    const double h1 = (2.0 * X0 + X1) / (X1 + X2);
-    const double h2 = (X1 + 2.0 * X2) / (X0 + X1);
-    double gai[5];
-    gai[0] = X0 / (X0 + X1);
-    gai[1] = 1.0 / (Xm1 + X0 + X1 + X2);
-    gai[2] = (2.0 * X0 * X1) / (X0 + X1) * ((Xm1 + X0) / (2.0 * X0 + X1) - (X2 + X1) / (2.0 * X1 + X0));
-    gai[3] = X0 * (Xm1 + X0) / (2.0 * X0 + X1);
-    gai[4] = X1 * (X1 + X2) / (X0 + 2.0 * X1);
-    double dma_j1[RFB], ai_j[RFB];
+  if (active) {
+    const double s0 = src[0], s1 = src[16];
+    // dma(0) factors: dx(-1), dx(0), dx(1) = s1, s0, s0
+    const double g0 = s0 / (s1 + s0 + s0);
+    const double g1 = (2.0 * s1 + s0) / (s0 + s0);
+    const double g2 = (s0 + 2.0 * s0) / (s1 + s0);
 #pragma unroll
     for (int i = 0; i < RFB; ++i) {
       if (!on[i]) continue;
-      dma_j1[i] = ppm_dma_g(A0[i], A1[i], A2[i], h0, h1, h2);            // dma(j+1)
-      ai_j[i] = ppm_ai_g(A0[i], A1[i], dma_j[i], dma_j1[i], gai);       // ai(j)
+      const double x0 = var[i][0], x1 = var[i][vs[i]];
+      const double v0 = state[i] ? x0 * s0 : x0, v1 = state[i] ? x1 * s1 : x1;
+      const double a1 = v0 / s0, a2 = v1 / s1;
+      const double am1 = alg == 3 ? a1 : a2;  // a(-1)
+      A0[i] = a1;  // a(0) = a(1)
+      A1[i] = a1;
+      A2[i] = a2;
+      V0[i] = 0.0;
+      V1[i] = v0;
+      V2[i] = v1;
+      dma_j[i] = ppm_dma_g(am1, A0[i], A1[i], g0, g1, g2);  // dma(0)
+      ai_prev[i] = 0.0;
+      var_prefix[i] = 0.0;
+      mass_prev[i] = 0.0;
     }
-    if (j >= 1) {
-      const int c = j - 1;  // physical source cell, Fortran cell j
+    fetch(3, pv);
+    fetch(4, qv);
+  }
+
+  // ------------------------------------------------------------ sweep
+  int k = 0;  // next target level
+  bool live = active;
+  for (int j0 = 0; j0 <= L; j0 += RCH) {
+    __syncthreads();  // the previous chunk's grid values are consumed (and the target table is written)
+    {
+      const int j = j0 + row;
+      if (row < RCH && j <= L) {
+        // window dx(j-1..j+2): dma(j+1) factors use cells j..j+2, ai(j) factors cells j-1..j+2
+        const double Xm1 = dxf(j - 1), X0 = dxf(j), X1 = dxf(j + 1), X2 = dxf(j + 2);
+        double *g = gsh + row * RG * 16 + pt;
+        g[0] = X1 / (X0 + X1 + X2);
+        g[16] = (2.0 * X0 + X1) / (X1 + X2);
+        g[32] = (X1 + 2.0 * X2) / (X0 + X1);
+        g[48] = X0 / (X0 + X1);
+        g[64] = 1.0 / (Xm1 + X0 + X1 + X2);
+        g[80] = (2.0 * X0 * X1) / (X0 + X1) * ((Xm1 + X0) / (2.0 * X0 + X1) - (X2 + X1) / (2.0 * X1 + X0));
+        g[96] = X0 * (Xm1 + X0) / (2.0 * X0 + X1);
+        g[112] = X1 * (X1 + X2) / (X0 + 2.0 * X1);
+        g[128] = X0;
+        g[144] = dxf(j + 3);  // the cell entering the window (used for j + 3 <= L)
+      }
+    }
+    __syncthreads();
+    if (!live) continue;
+    const int jn = L + 1 - j0 < RCH ? L + 1 - j0 : RCH;
+    for (int jj = 0; jj < jn; ++jj) {
+      const int j = j0 + jj;
+      double nv[RFB];
+      fetch(j + 5, nv);
+      const double *g = gsh + jj * RG * 16 + pt;
+      const double h0 = g[0], h1 = g[16], h2 = g[32];
+      const double gai[5] = {g[48], g[64], g[80], g[96], g[112]};
+      const double X0 = g[128];
+      double dma_j1[RFB], ai_j[RFB];
 #pragma unroll
       for (int i = 0; i < RFB; ++i) {
         if (!on[i]) continue;
-        ppm_parabola(ai_prev[i], ai_j[i], A0[i], cc0[i], cc1[i], cc2[i]);
-        if (alg == 2 && (c <= 1 || c >= L - 2)) {
-          cc0[i] = A0[i];
-          cc1[i] = 0.0;
-          cc2[i] = 0.0;
-        }
+        dma_j1[i] = ppm_dma_g(A0[i], A1[i], A2[i], h0, h1, h2);       // dma(j+1)
+        ai_j[i] = ppm_ai_g(A0[i], A1[i], dma_j[i], dma_j1[i], gai);  // ai(j)
       }
-      // kid is non-decreasing in k (each search continues from the last
-      // interface reached, starting no lower than k), so every target is
-      // finished while the sweep is at its own cell kid(k)
-      while (k < L) {
-        if (kid == -2) compute_kid();
-        if (kid == c) {
-          finish(X0);
-        } else if (kid < c) {
-          atomicOr(a.error, 2);  // deformation beyond one layer: not supported
-          return;
+      if (j >= 1) {
+        const int c = j - 1;  // physical source cell, Fortran cell j
+#pragma unroll
+        for (int i = 0; i < RFB; ++i) {
+          if (!on[i]) continue;
+          ppm_parabola(ai_prev[i], ai_j[i], A0[i], cc0[i], cc1[i], cc2[i]);
+          if (alg == 2 && (c <= 1 || c >= L - 2)) {
+            cc0[i] = A0[i];
+            cc1[i] = 0.0;
+            cc2[i] = 0.0;
+          }
+        }
+        // kid is non-decreasing in k (each search continues from the last
+        // interface reached, starting no lower than k), so every target is
+        // finished while the sweep is at its own cell kid(k)
+        while (k < L) {
+          const int kid = ksh[k * 16 + pt];
+          if (kid > c) break;
+          if (kid < c) {
+            atomicOr(a.error, 2);  // deformation beyond one layer: not supported
+            live = false;
+            break;
+          }
+          const double z2 = zsh[k * 16 + pt];
+          const double tk = dab[2 * k] * a.ps0 + dab[2 * k + 1] * ps;  // the reference's (hyai(k+1)-hyai(k))*ps0 + ...
+#pragma unroll
+          for (int i = 0; i < RFB; ++i) {
+            if (!on[i]) continue;
+            const double integral = integrate_parabola(cc2[i], cc1[i], cc0[i], -0.5, z2);
+            const double mass = var_prefix[i] + integral * X0;
+            const double out = k > 0 ? mass - mass_prev[i] : mass;
+            mass_prev[i] = mass;
+            if (state[i]) {
+              var[i][(size_t)k * vs[i]] = out / tk;
+            } else {
+              var[i][(size_t)k * vs[i]] = out;
+              if (qout[i]) ((__attribute__((address_space(1))) double *)Qout[i])[(size_t)k * 16] = out / tk;
+            }
+          }
+          ++k;
+        }
+        if (!live) break;
+#pragma unroll
+        for (int i = 0; i < RFB; ++i)
+          if (on[i]) var_prefix[i] += V0[i];
+      }
+      // shift: bring in cell j+3
+      const int m = j + 3;
+      const double xn = g[144];
+#pragma unroll
+      for (int i = 0; i < RFB; ++i) {
+        if (!on[i]) continue;
+        double an, vn = 0.0;
+        if (m <= L) {
+          const double x = pv[i];
+          vn = state[i] ? x * xn : x;
+          an = vn / xn;
+        } else if (m == L + 1) {
+          an = A2[i];  // a(L+1) = a(L)
         } else {
-          break;
+          an = alg == 3 ? A1[i] : A0[i];  // a(L+2) = a(L) or a(L-1)
         }
+        A0[i] = A1[i]; A1[i] = A2[i]; A2[i] = an;
+        V0[i] = V1[i]; V1[i] = V2[i]; V2[i] = vn;
+        ai_prev[i] = ai_j[i];
+        dma_j[i] = dma_j1[i];
+        pv[i] = qv[i];
+        qv[i] = nv[i];
       }
-#pragma unroll
-      for (int i = 0; i < RFB; ++i)
-        if (on[i]) var_prefix[i] += V0[i];
     }
-    // shift: bring in cell j+3
-    const int m = j + 3;
-    double xn;
-    if (m <= L) xn = px;
-    else if (m == L + 1) xn = X2;  // dx(L+1) = dx(L)
-    else xn = X0;                  // dx(L+2) = dx(L-1)
-#pragma unroll
-    for (int i = 0; i < RFB; ++i) {
-      if (!on[i]) continue;
-      double an, vn = 0.0;
-      if (m <= L) {
-        const double x = pv[i];
-        vn = state[i] ? x * xn : x;
-        an = vn / xn;
-      } else if (m == L + 1) {
-        an = A2[i];  // a(L+1) = a(L)
-      } else {
-        an = alg == 3 ? A1[i] : A0[i];  // a(L+2) = a(L) or a(L-1)
-      }
-      Am1[i] = A0[i]; A0[i] = A1[i]; A1[i] = A2[i]; A2[i] = an;
-      V0[i] = V1[i]; V1[i] = V2[i]; V2[i] = vn;
-      ai_prev[i] = ai_j[i];
-      dma_j[i] = dma_j1[i];
-      pv[i] = qv[i];
-      qv[i] = nv[i];
-    }
-    Xm1 = X0; X0 = X1; X1 = X2; X2 = xn;
-    px = qx;
-    qx = nx;
   }
-  if (k < L) atomicOr(a.error, 2);
+  if (live && k < L) atomicOr(a.error, 2);
 }
 
 __global__ void zero_int_kernel(int *p) { *p = 0; }
@@ -342,14 +380,19 @@ void launch_zero_int(hipStream_t st, int *p) { hipLaunchKernelGGL(zero_int_kerne
 void launch_remap(hipStream_t st, const RemapArgs &a) {
   const size_t ncol = (size_t)a.nelem * 16;
   hipLaunchKernelGGL(remap_thickness_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, a);
-  // four fields per lane: the column work (grid factors, target search,
-  // control flow) is shared four ways at two waves per SIMD
-  constexpr int fb = 4;
-  const int nfb = (a.nstates + a.qsize + fb - 1) / fb;
-  if (nfb == 0) return;
-  const int nwaves = ((a.nelem + 3) / 4) * nfb;
-  const int nb = (((nwaves + 3) / 4) + 7) & ~7;
-  hipLaunchKernelGGL(remap_kernel<fb>, dim3(nb), dim3(256), 0, st, a, nfb);
+  const int nfld = a.nstates + a.qsize;
+  if (nfld == 0) return;
+  // fields per lane: as few as fill the 16 rows (at most 4, then more
+  // blocks per element)
+  const int rfb = nfld <= 16 ? 1 : nfld <= 32 ? 2 : nfld <= 48 ? 3 : 4;
+  const dim3 g(a.nelem, (nfld + 16 * rfb - 1) / (16 * rfb));
+  const size_t lds = remap_lds_bytes(a.nlev);
+  switch (rfb) {
+    case 1: hipLaunchKernelGGL(remap_kernel<1>, g, dim3(256), lds, st, a); break;
+    case 2: hipLaunchKernelGGL(remap_kernel<2>, g, dim3(256), lds, st, a); break;
+    case 3: hipLaunchKernelGGL(remap_kernel<3>, g, dim3(256), lds, st, a); break;
+    default: hipLaunchKernelGGL(remap_kernel<4>, g, dim3(256), lds, st, a); break;
+  }
 }
 
 }  // namespace hx

@@ -106,7 +106,7 @@ void f90_push_forcing_to_cxx(double *FM, double *FT, double *FQ, double *Qdp);
 
 /* nlev and qsize_d are compile-time constants in the reference (PLEV,
  * QSIZE_D); set them before init_elements_2d_c.  Defaults 72 and qsize.
- * Returns 0, 1 if elements already exist, 2 if nlev is out of range. */
+ * Returns 0, 1 if elements already exist, 2 if nlev is outside 3..256. */
 int hommexx_hip_configure(int nlev, int qsize_d);
 /* 1 (default): abort like the reference; 0: record the error and return */
 void hommexx_hip_set_abort_on_error(int on);

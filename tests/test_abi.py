@@ -49,3 +49,15 @@ def test_header_compiles(tmp_path, lang, compiler):
     src = tmp_path / ("t." + ("c" if lang == "c" else "cpp"))
     src.write_text('#include "hommexx_amd.h"\nint main(void) { return 0; }\n')
     subprocess.check_call([compiler, "-fsyntax-only", "-I", os.path.dirname(HEADER), str(src)])
+
+
+def test_configure_bounds_nlev():
+    """hommexx_hip_configure accepts 3 <= nlev <= 256 (the remap keeps each
+    element's target table in LDS) and touches no GPU (separate process, so
+    the library's session state stays private)."""
+    if not os.path.exists(LIB):
+        pytest.skip("libhommexx_amd.so not built")
+    code = ("import ctypes, sys; L = ctypes.CDLL(sys.argv[1]); "
+            "print([L.hommexx_hip_configure(n, 4) for n in (2, 3, 72, 256, 257, 4096)])")
+    out = subprocess.check_output([os.sys.executable, "-c", code, LIB]).decode().strip()
+    assert out == "[2, 0, 0, 0, 2, 2]", out
