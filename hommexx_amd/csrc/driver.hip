@@ -84,6 +84,11 @@ struct Ctx {
   double *qdp_pre2 = nullptr, *fdss_pre2 = nullptr;  // stage-2 pre-exchange buffers
   int *err_d = nullptr;
   int *err_h = nullptr;
+  // diagnostics: the Fortran arrays of init_diagnostics_c (q, qvar, qmass,
+  // q1mass, iener, iener_wet, kener, pener) and a device staging buffer
+  double *dg[8] = {};
+  bool dg_set = false;
+  double *dg_dev = nullptr;
   // time levels (0-based)
   int nm1 = 0, n0 = 1, np1 = 2, nstep = 0, nstep0 = 0, n0_qdp = 0, np1_qdp = 1;
   // 0: FP64 step; 1: mixed form (FP32 sphere operators in hyperviscosity and
@@ -161,6 +166,7 @@ void free_all() {
   c.timers.clear();
   if (c.err_h) hipHostFree(c.err_h);
   c.err_h = nullptr;
+  c.dg_dev = nullptr;
 }
 
 // ---------------------------------------------------------------- timing
@@ -814,6 +820,85 @@ void push_tracers(double *f90, const double *dev, int ntl_f, int tl) {
              e3 * sizeof(double));
 }
 
+// ------------------------------------------------------------ diagnostics
+// Diagnostics.cpp:37-185.  The device sums land in a staging buffer; slot
+// ivar of each Fortran array is written with strided copies (the other slots
+// keep their values, as in the reference).
+double *diag_staging() {
+  Ctx &c = C();
+  if (!c.dg_dev) c.dg_dev = dalloc<double>((size_t)c.nelem * 16 * (4 + 2 * (size_t)c.qsize));
+  return c.dg_dev;
+}
+
+// slot ivar of a Fortran array [ie][nslot][ncol] from a device array
+// [ie][ncol] (ncol doubles per element, the first ncol of a row of width
+// row_f doubles in the Fortran array)
+void diag_copy(double *f90, int nslot, int ivar, size_t row_f, const double *dev, size_t ncol) {
+  Ctx &c = C();
+  hipMemcpy2DAsync(f90 + (size_t)ivar * row_f, (size_t)nslot * row_f * sizeof(double), dev, ncol * sizeof(double),
+                   ncol * sizeof(double), c.nelem, hipMemcpyDeviceToHost, c.st);
+}
+
+DiagArgs diag_args(double *out) {
+  Ctx &c = C();
+  DiagArgs a{};
+  a.nelem = c.nelem;
+  a.nlev = c.nlev;
+  a.qsize = c.qsize;
+  a.use_cpstar = c.p.use_cpstar && c.qsize > 0;
+  a.phis = c.phis;
+  a.Q = c.Q;
+  a.hyai = c.hyai_d;
+  a.hybi = c.hybi_d;
+  a.ps0 = c.ps0;
+  a.out = out;
+  return a;
+}
+
+// prim_diag_scalars (Diagnostics.cpp:37-88): Q to the Fortran Q array, then
+// per column and tracer sum(Qdp*Q) -> qvar[slot], sum(Qdp) -> qmass[slot] and
+// q1mass, at tracer level n0_qdp (before the advance) or np1_qdp (after)
+void diag_scalars(bool before_advance, int ivar) {
+  Ctx &c = C();
+  if (!c.dg_set || c.p.time_step_type <= 0) return;
+  double *out = diag_staging();
+  const size_t e3 = (size_t)c.nlev * 16, nq = (size_t)c.nelem * c.qsize * 16;
+  hipStreamSynchronize(c.st);
+  push_tracers(c.dg[0], c.Q, 1, 0);
+  for (int ie = 0; ie < c.nelem; ++ie)
+    for (int q = c.qsize; q < c.qsize_d; ++q) memset(c.dg[0] + ((size_t)ie * c.qsize_d + q) * e3, 0, e3 * sizeof(double));
+  if (c.qsize == 0) return;
+  DiagArgs a = diag_args(out);
+  a.qdp = c.qdp[before_advance ? c.n0_qdp : c.np1_qdp];
+  launch_diag_scalars(c.st, a);
+  const size_t row = (size_t)c.qsize_d * 16, ncol = (size_t)c.qsize * 16;
+  diag_copy(c.dg[1], 4, ivar, row, out, ncol);       // qvar
+  diag_copy(c.dg[2], 4, ivar, row, out + nq, ncol);  // qmass
+  diag_copy(c.dg[3], 1, 0, row, out + nq, ncol);     // q1mass
+  hipok(hipStreamSynchronize(c.st), "prim_diag_scalars");
+}
+
+// prim_energy_halftimes (Diagnostics.cpp:90-185) at dynamics level n0 (before
+// the advance) or np1 (after): iener, kener, pener [slot] and iener_wet
+void diag_energy(bool before_advance, int ivar) {
+  Ctx &c = C();
+  if (!c.dg_set) return;
+  double *out = diag_staging();
+  const size_t n = (size_t)c.nelem * 16;
+  const int tl = before_advance ? c.n0 : c.np1;
+  DiagArgs a = diag_args(out);
+  a.v = c.v[tl];
+  a.t = c.t[tl];
+  a.ps = c.ps[tl];
+  a.qdp = c.qdp[before_advance ? c.n0_qdp : c.np1_qdp];
+  launch_diag_energy(c.st, a);
+  diag_copy(c.dg[4], 4, ivar, 16, out, 16);          // iener
+  diag_copy(c.dg[5], 1, 0, 16, out + n, 16);         // iener_wet
+  diag_copy(c.dg[6], 4, ivar, 16, out + 2 * n, 16);  // kener
+  diag_copy(c.dg[7], 4, ivar, 16, out + 3 * n, 16);  // pener
+  hipok(hipStreamSynchronize(c.st), "prim_energy_halftimes");
+}
+
 }  // namespace
 
 // =================================================================== C-ABI
@@ -1108,8 +1193,19 @@ void init_elements_states_c(const double *const &v, const double *const &T, cons
   c.states_set = true;
 }
 
-// cxx_f90_interface.cpp:253-262 (diagnostics are disabled on this path)
-void init_diagnostics_c(double *&, double *&, double *&, double *&, double *&, double *&, double *&, double *&) {}
+// cxx_f90_interface.cpp:253-262, Diagnostics.cpp:17-35: the Fortran arrays
+// prim_run_subcycle_c fills when diagnostics are enabled
+void init_diagnostics_c(double *&q, double *&qvar, double *&qmass, double *&q1mass, double *&iener,
+                        double *&iener_wet, double *&kener, double *&pener) {
+  Ctx &c = C();
+  double *p[8] = {q, qvar, qmass, q1mass, iener, iener_wet, kener, pener};
+  bool all = true;
+  for (int i = 0; i < 8; ++i) {
+    c.dg[i] = p[i];
+    all = all && p[i] != nullptr;
+  }
+  c.dg_set = all;
+}
 
 // cxx_f90_interface.cpp:264-280
 void init_boundary_exchanges_c() {
@@ -1138,13 +1234,22 @@ void init_time_level_c(const int &nm1, const int &n0, const int &np1, const int 
 
 // prim_driver.cpp:31-156: one remap interval.  Indices come back 0-based.
 void prim_run_subcycle_c(const double &dt, int &nstep, int &nm1, int &n0, int &np1, const int &last_time_step) {
-  (void)last_time_step;
   Ctx &c = C();
   if (!ready("prim_run_subcycle_c")) return;
   if (!c.be_set) init_boundary_exchanges_c();
   const double dt_q = dt * c.p.qsplit;
   const double dt_remap = c.p.rsplit > 0 ? dt_q * c.p.rsplit : dt_q;
+  // prim_driver.cpp:42-63: diagnostics at the state frequency, the first and
+  // the last step, unless disabled (or no arrays were registered)
+  const int nstep_end = c.nstep + c.p.qsplit * (c.p.rsplit > 0 ? c.p.rsplit : 1);
+  const bool diag = !c.p.disable_diagnostics && c.dg_set &&
+                    ((c.p.state_frequency > 0 && nstep_end % c.p.state_frequency == 0) || nstep_end == c.nstep0 ||
+                     nstep_end >= last_time_step);
   update_tracers_levels();
+  if (diag) {
+    diag_scalars(true, 3);
+    diag_energy(true, 2);
+  }
   if (c.p.ftype == 0) {
     Scope s("forcing");
     launch_state_forcing(c.st, c.nelem, c.nlev, dt_remap, c.ft, c.fm, c.t[c.n0], c.v[c.n0]);
@@ -1153,6 +1258,10 @@ void prim_run_subcycle_c(const double &dt, int &nstep, int &nm1, int &n0, int &n
   } else if (c.p.ftype == 2) {
     Scope s("forcing");
     launch_state_forcing(c.st, c.nelem, c.nlev, dt_remap, c.ft, c.fm, c.t[c.n0], c.v[c.n0]);
+  }
+  if (diag) {
+    diag_energy(true, 0);
+    diag_scalars(true, 0);
   }
   {
     Scope s("dp3d_from_ps");
@@ -1171,6 +1280,10 @@ void prim_run_subcycle_c(const double &dt, int &nstep, int &nm1, int &n0, int &n
   update_tracers_levels();
   launch_zero_int(c.st, c.err_d);
   remap(dt_remap);
+  if (diag) {
+    diag_scalars(false, 1);
+    diag_energy(false, 1);
+  }
   leapfrog();
   nstep = c.nstep;
   nm1 = c.nm1;

@@ -168,6 +168,18 @@ void launch_state_forcing(hipStream_t st, int nelem, int nlev, double dt, const 
                           double *t, double *v);
 void launch_tracer_forcing(hipStream_t st, int nelem, int nlev, int qsize, bool moist, double dt, const double *fq,
                            double *qdp, double *Q, const double *hyai, const double *hybi, double ps0, double *ps);
+// Diagnostics (Diagnostics.cpp:37-185): energy sums of dynamics level
+// (v, t, ps) and, with use_cpstar, tracer 0 of qdp; tracer sums of qdp and Q.
+struct DiagArgs {
+  int nelem, nlev, qsize, use_cpstar;
+  const double *v, *t, *ps, *phis, *qdp, *Q;
+  const double *hyai, *hybi;  // nlev+1
+  double ps0;
+  double *out;  // energy: [4][nelem*16]; scalars: [2][nelem*qsize*16]
+};
+void launch_diag_energy(hipStream_t st, const DiagArgs &a);
+void launch_diag_scalars(hipStream_t st, const DiagArgs &a);
+
 // largest nlev of the remap kernel: an element's target table (10 B per
 // level and column) and a grid chunk (20 KB) fit in 64 KB of LDS
 constexpr int REMAP_MAX_NLEV = 256;

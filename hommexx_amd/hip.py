@@ -16,6 +16,8 @@ import os
 
 import numpy as np
 
+from .problem import DIAGNOSTIC_ARRAYS, diagnostic_arrays
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhommexx_amd.so")
 
@@ -128,7 +130,15 @@ class HipSession:
         st = [f64(s[k]) for k in ("v", "T", "dp3d", "Qdp", "ps_v")]
         L.init_elements_states_c(*[_a(x) for x in st])
         self._check("init_elements_states_c")
-        diag = [np.zeros(1) for _ in range(8)]
+        # init_diagnostics_c: the arrays prim_run_subcycle_c fills when
+        # diagnostics are enabled (placeholders otherwise)
+        self.diag = None
+        if p.disable_diagnostics:
+            diag = [np.zeros(1) for _ in range(8)]
+        else:
+            self.diag = diagnostic_arrays(prob)
+            diag = [self.diag[k] for k in DIAGNOSTIC_ARRAYS]
+        self._keep += diag
         L.init_diagnostics_c(*[_a(x) for x in diag])
         vc = prob.vcoord
         hy = [f64(vc.hyam), f64(vc.hyai), f64(vc.hybm), f64(vc.hybi)]
@@ -154,10 +164,10 @@ class HipSession:
             self.L.hommexx_hip_clear_error()
             raise HommexxError("%s failed (%d): %s" % (what, code, msg))
 
-    def run(self, nsteps=1, dt=None):
+    def run(self, nsteps=1, dt=None, last_time_step=1 << 30):
         dt = self.prob.dt if dt is None else dt
         nstep, nm1, n0, np1 = _I(), _I(), _I(), _I()
-        last = _I(1 << 30)
+        last = _I(last_time_step)
         for _ in range(nsteps):
             self.L.prim_run_subcycle_c(ctypes.byref(_D(dt)), ctypes.byref(nstep), ctypes.byref(nm1),
                                        ctypes.byref(n0), ctypes.byref(np1), ctypes.byref(last))
@@ -174,6 +184,10 @@ class HipSession:
                                        _a(out["Q"]), _a(out["ps_v"]), _a(out["omega_p"]))
         self._check("cxx_push_results_to_f90")
         return out
+
+    def diagnostics(self):
+        """Copies of the diagnostic arrays (None with disable_diagnostics)."""
+        return None if self.diag is None else {k: v.copy() for k, v in self.diag.items()}
 
     def synchronize(self):
         self.L.hommexx_hip_synchronize()

@@ -110,3 +110,20 @@ def make_forcing(prob, seed=0, ft_scale=1e-4, fm_scale=1e-5, fq_scale=1e-7):
         c = rng.standard_normal(4)
         FQ[:, q] = fq_scale * (c[0] + c[1] * x + c[2] * y + c[3] * z)[:, None] * kk
     return {"FM": np.ascontiguousarray(FM), "FT": np.ascontiguousarray(FT), "FQ": np.ascontiguousarray(FQ)}
+
+
+# init_diagnostics_c's arrays in argument order (Diagnostics.cpp:17-35):
+# the Fortran elem%state%Q and elem%accum arrays, here with C shapes (last
+# Fortran index first, np x np trailing)
+DIAGNOSTIC_ARRAYS = ("q", "qvar", "qmass", "q1mass", "iener", "iener_wet", "kener", "pener")
+
+
+def diagnostic_arrays(prob):
+    """Zeroed diagnostic arrays for ``prob``: q (nelem, qsize_d, nlev, 4, 4),
+    qvar and qmass (nelem, 4, qsize_d, 4, 4) with the slot index 0..3 second,
+    q1mass (nelem, qsize_d, 4, 4), iener, kener, pener (nelem, 4, 4, 4) and
+    iener_wet (nelem, 4, 4)."""
+    E, L, qd = prob.nelem, prob.nlev, prob.qsize_d
+    shapes = {"q": (E, qd, L, 4, 4), "qvar": (E, 4, qd, 4, 4), "qmass": (E, 4, qd, 4, 4), "q1mass": (E, qd, 4, 4),
+              "iener": (E, 4, 4, 4), "iener_wet": (E, 4, 4), "kener": (E, 4, 4, 4), "pener": (E, 4, 4, 4)}
+    return {k: np.zeros(shapes[k]) for k in DIAGNOSTIC_ARRAYS}

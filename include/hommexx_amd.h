@@ -74,7 +74,11 @@ void init_elements_2d_c(HX_REF(const int) num_elems, HX_REF(const double *const)
 void init_elements_states_c(HX_REF(const double *const) v, HX_REF(const double *const) T,
                             HX_REF(const double *const) dp3d, HX_REF(const double *const) qdp,
                             HX_REF(const double *const) ps_v);
-/* src/share/cxx/cxx_f90_interface.cpp:253-262 (diagnostics are disabled on this path) */
+/* src/share/cxx/cxx_f90_interface.cpp:253-262, Diagnostics.cpp:17-35: the
+ * Fortran arrays q (np,np,nlev,qsize_d), qvar and qmass (np,np,qsize_d,4),
+ * q1mass (np,np,qsize_d), iener, kener, pener (np,np,4), iener_wet (np,np),
+ * per element; prim_run_subcycle_c fills them on the reference's diagnostic
+ * steps unless disable_diagnostics (prim_driver.cpp:49-63,84-88,140-144) */
 void init_diagnostics_c(HX_REF(double *) q, HX_REF(double *) qvar, HX_REF(double *) qmass, HX_REF(double *) q1mass,
                         HX_REF(double *) iener, HX_REF(double *) iener_wet, HX_REF(double *) kener,
                         HX_REF(double *) pener);

@@ -57,6 +57,8 @@ def _bind(path):
     L.oracle_num_threads.restype = ctypes.c_int
     L.oracle_debug_stop.argtypes = [_vp, ctypes.c_int]
     L.oracle_set_forcing.argtypes = [_vp, _dp, _dp, _dp]
+    L.oracle_set_diagnostics.argtypes = [_vp, ctypes.c_int, ctypes.c_int] + [_dp] * 8
+    L.oracle_set_last_time_step.argtypes = [_vp, ctypes.c_int]
     return L
 
 
@@ -126,11 +128,17 @@ class OracleSession:
             f = prob.forcing
             self._keep["forcing"] = [f64(f["FM"]), f64(f["FT"]), f64(f["FQ"])]
             L.oracle_set_forcing(self.h, *[ptr(a) for a in self._keep["forcing"]])
+        self.diag = None
+        if not p.disable_diagnostics:
+            from hommexx_amd.problem import DIAGNOSTIC_ARRAYS, diagnostic_arrays
+            self.diag = diagnostic_arrays(prob)
+            L.oracle_set_diagnostics(self.h, 0, int(p.use_cpstar), *[ptr(self.diag[k]) for k in DIAGNOSTIC_ARRAYS])
         self.tl = None
 
-    def run(self, nsteps=1, dt=None):
+    def run(self, nsteps=1, dt=None, last_time_step=1 << 30):
         L = self.L
         dt = self.prob.dt if dt is None else dt
+        L.oracle_set_last_time_step(self.h, last_time_step)
         a = [ctypes.c_int() for _ in range(4)]
         for _ in range(nsteps):
             rc = L.oracle_run_subcycle(self.h, dt, *[ctypes.byref(x) for x in a])
@@ -147,6 +155,10 @@ class OracleSession:
         self.L.oracle_get_states(self.h, ptr(out["v"]), ptr(out["T"]), ptr(out["dp3d"]), ptr(out["Qdp"]),
                                 ptr(out["Q"]), ptr(out["ps_v"]), ptr(out["omega_p"]))
         return out
+
+    def diagnostics(self):
+        """Copies of the diagnostic arrays (None with disable_diagnostics)."""
+        return None if self.diag is None else {k: v.copy() for k, v in self.diag.items()}
 
     def close(self):
         if self.h:

@@ -18,6 +18,8 @@
  *   Euler tracer step   EulerStepFunctorImpl.hpp:196-561 (team limiters :693-884)
  *   vertical remap      RemapFunctor.hpp:226-467, PpmRemap.hpp:78-673
  *   DSS                 mpi/BoundaryExchange.cpp:285-849, ConnectivityHelpers.hpp
+ *   diagnostics         Diagnostics.cpp:37-185, called as prim_driver.cpp:49-63,
+ *                       84-88, 140-144 place them
  * Floating-point operations are written in the reference's order (left to
  * right, serial reductions and scans in index order, as the reference does on
  * CPU and with HOMMEXX_GPU_BFB_WITH_CPU on GPU).  Build with
@@ -43,6 +45,11 @@
  *                     initial mass plus the clipped dt*FQ; a uniform FT raises
  *                     T by dt*FT
  *   full step         dry and tracer mass conserved to 1e-14
+ *   diagnostics       PEner = phis (ps_v - hyai(0) ps0 + hyai(L) ps0) to
+ *                     rounding; Qmass summed over the sphere conserved from
+ *                     slot 3 (before) to slot 1 (after); Qvar = sum Qdp^2/dp;
+ *                     IEner_wet = 0 without cp*; the slot schedule of
+ *                     state_frequency, nstep0 and the last step
  * Each check is shown to fail on a build with one planted mistake
  * (ORACLE_PLANT below).  The exact accumulation order of the DSS follows
  * ConnectivityHelpers.hpp; property checks cannot see summation order, so
@@ -76,6 +83,7 @@
 static const double RGAS = 287.04;
 static const double CP = 1005.0;
 static const double RWATER_VAPOR = 461.5;
+static const double CPWATER_VAPOR = 1870.0;
 #define KAPPA (RGAS / CP)
 #define RREARTH (1.0 / 6.376e6)
 
@@ -107,6 +115,9 @@ typedef struct {
   int dss_buf_fields;
   /* time levels */
   int nm1, n0, np1, nstep, nstep0, n0_qdp, np1_qdp;
+  /* diagnostics (Fortran elem%state%Q and elem%accum arrays, flat) */
+  int diag_on, use_cpstar, last_time_step;
+  double *dg_q, *dg_qvar, *dg_qmass, *dg_q1mass, *dg_iener, *dg_iener_wet, *dg_kener, *dg_pener;
   int error;
   int debug_stop, stopped;
 } Oracle;
@@ -1285,6 +1296,8 @@ Oracle *oracle_create(int nelem, int nlev, int qsize, int qsize_d) {
   o->nlev = nlev;
   o->qsize = qsize;
   o->qsize_d = qsize_d < qsize ? qsize : qsize_d;
+  o->last_time_step = 1 << 30;
+  o->disable_diagnostics = 1;
   const size_t e3 = (size_t)NP2 * nlev, ne = nelem;
   o->D = alloc0(ne * 64); o->Dinv = alloc0(ne * 64); o->metinv = alloc0(ne * 64);
   o->tensorvisc = alloc0(ne * 64); o->vec_sph2cart = alloc0(ne * 96);
@@ -1517,6 +1530,86 @@ void oracle_set_time_level(Oracle *o, int nm1, int n0, int np1, int nstep, int n
   o->nm1 = nm1 - 1; o->n0 = n0 - 1; o->np1 = np1 - 1; o->nstep = nstep; o->nstep0 = nstep0;
 }
 
+/* ------------------------------------------------------------ diagnostics
+ * Diagnostics::init (Diagnostics.cpp:17-35): the Fortran arrays, C views
+ *   q        [ie][QSIZE_D][lev][igp][jgp]   qvar, qmass [ie][4][QSIZE_D][igp][jgp]
+ *   q1mass   [ie][QSIZE_D][igp][jgp]        iener, kener, pener [ie][4][igp][jgp]
+ *   iener_wet [ie][igp][jgp]
+ * disable_diagnostics as the C-ABI passes it; last_time_step as
+ * prim_run_subcycle_c receives it (oracle_set_last_time_step). */
+void oracle_set_diagnostics(Oracle *o, int disable_diagnostics, int use_cpstar, double *q, double *qvar,
+                            double *qmass, double *q1mass, double *iener, double *iener_wet, double *kener,
+                            double *pener) {
+  o->disable_diagnostics = disable_diagnostics;
+  o->use_cpstar = use_cpstar;
+  o->diag_on = !disable_diagnostics && q && qvar && qmass && q1mass && iener && iener_wet && kener && pener;
+  o->dg_q = q; o->dg_qvar = qvar; o->dg_qmass = qmass; o->dg_q1mass = q1mass;
+  o->dg_iener = iener; o->dg_iener_wet = iener_wet; o->dg_kener = kener; o->dg_pener = pener;
+}
+
+void oracle_set_last_time_step(Oracle *o, int last_time_step) { o->last_time_step = last_time_step; }
+
+/* Diagnostics::prim_diag_scalars (Diagnostics.cpp:37-88): the device Q goes to
+   the Fortran Q array, then per column and tracer the level sums (in level
+   order) of Qdp*Q and of Qdp at the tracer level n0_qdp (before the advance)
+   or np1_qdp (after). */
+static void prim_diag_scalars(Oracle *o, int before_advance, int ivar) {
+  const size_t e3 = E3;
+  const int qd = o->qsize_d, t2 = before_advance ? o->n0_qdp : o->np1_qdp;
+  for (int ie = 0; ie < o->nelem; ++ie) {
+    for (int q = 0; q < qd; ++q) {
+      double *f = o->dg_q + ((size_t)ie * qd + q) * e3;
+      if (q < o->qsize) o2f3(o->Q + ((size_t)ie * o->qsize + q) * e3, f, L);
+      else memset(f, 0, sizeof(double) * e3);
+    }
+    for (int q = 0; q < o->qsize; ++q)
+      for (int p = 0; p < NP2; ++p) {
+        const double *qdp = QDP(ie, t2, q) + (size_t)p * L;
+        const double *Qf = o->dg_q + ((size_t)ie * qd + q) * e3 + p;
+        double acc_qdp_q = 0, acc_qdp = 0;
+        for (int k = 0; k < L; ++k) {
+          acc_qdp_q += qdp[k] * Qf[(size_t)k * NP2];
+          acc_qdp += qdp[k];
+        }
+        o->dg_qvar[(((size_t)ie * 4 + ivar) * qd + q) * NP2 + p] = acc_qdp_q;
+        o->dg_qmass[(((size_t)ie * 4 + ivar) * qd + q) * NP2 + p] = acc_qdp;
+        o->dg_q1mass[((size_t)ie * qd + q) * NP2 + p] = acc_qdp;
+      }
+  }
+}
+
+/* Diagnostics::prim_energy_halftimes (Diagnostics.cpp:90-185): per column,
+   level sums of cp* T dp, (cp* - cp) T dp, (u^2 + v^2)/2 dp and phis dp at
+   the dynamics level n0 (before) or np1 (after), dp from ps_v of that level.
+   cp* = cp (1 + (Cpv/cp - 1) Qdp_1/dp) with use_cpstar, else cp.  std::pow(u,2)
+   is u*u (both are the correctly rounded square). */
+static void prim_energy_halftimes(Oracle *o, int before_advance, int ivar) {
+  const int t1 = before_advance ? o->n0 : o->np1, t1q = before_advance ? o->n0_qdp : o->np1_qdp;
+  for (int ie = 0; ie < o->nelem; ++ie)
+    for (int p = 0; p < NP2; ++p) {
+      const double *u = V(ie, t1, 0) + (size_t)p * L, *v = V(ie, t1, 1) + (size_t)p * L;
+      const double *t = T(ie, t1) + (size_t)p * L;
+      const double ps = PSV(ie, t1)[p], phis = o->phis[(size_t)ie * NP2 + p];
+      double ie_ = 0, iew = 0, ke = 0, pe = 0;
+      for (int k = 0; k < L; ++k) {
+        const double dpt1 = o->hyai_delta[k] * o->ps0 + o->hybi_delta[k] * ps;
+        double cps = CP;
+        if (o->use_cpstar) {
+          const double qv = QDP(ie, t1q, 0)[(size_t)p * L + k] / dpt1;
+          cps = CP * (1.0 + (CPWATER_VAPOR / CP - 1.0) * qv);
+        }
+        ie_ += cps * t[k] * dpt1;
+        iew += (cps - CP) * t[k] * dpt1;
+        ke += (u[k] * u[k] + v[k] * v[k]) * 0.5 * dpt1;
+        pe += (PLANT(13) ? 2.0 : 1.0) * phis * dpt1; /* plant 13: PEner doubled */
+      }
+      o->dg_iener[((size_t)ie * 4 + ivar) * NP2 + p] = ie_;
+      o->dg_iener_wet[(size_t)ie * NP2 + p] = iew;
+      o->dg_kener[((size_t)ie * 4 + ivar) * NP2 + p] = ke;
+      o->dg_pener[((size_t)ie * 4 + ivar) * NP2 + p] = pe;
+    }
+}
+
 /* prim_run_subcycle_c (prim_driver.cpp:31-156).  Returns 0, or a nonzero
    error code (1: negative or NaN layer thickness at the remap). */
 int oracle_run_subcycle(Oracle *o, double dt, int *nstep, int *nm1, int *n0, int *np1) {
@@ -1529,8 +1622,21 @@ int oracle_run_subcycle(Oracle *o, double dt, int *nstep, int *nm1, int *n0, int
   }
   const double dt_q = dt * o->qsplit;
   const double dt_remap = o->rsplit > 0 ? dt_q * o->rsplit : dt_q;
+  /* prim_driver.cpp:42-63: diagnostics at the state frequency, the first and
+     the last step, unless disabled */
+  const int nstep_end = o->nstep + o->qsplit * (o->rsplit > 0 ? o->rsplit : 1);
+  const int diag = o->diag_on && ((o->state_frequency > 0 && nstep_end % o->state_frequency == 0) ||
+                                  nstep_end == o->nstep0 || nstep_end >= o->last_time_step);
+  if (diag) {
+    prim_diag_scalars(o, 1, PLANT(14) ? 2 : 3); /* plant 14: slot of the first call */
+    prim_energy_halftimes(o, 1, 2);
+  }
   if (o->ftype == 0) cam_forcing(o, dt_remap, 1);
   else if (o->ftype == 2) cam_forcing(o, dt_remap, 0);
+  if (diag) {
+    prim_energy_halftimes(o, 1, 0);
+    prim_diag_scalars(o, 1, 0);
+  }
   /* dp3d from ps */
 #pragma omp parallel for schedule(static)
   for (int ie = 0; ie < o->nelem; ++ie)
@@ -1565,6 +1671,10 @@ int oracle_run_subcycle(Oracle *o, double dt, int *nstep, int *nm1, int *n0, int
             const double dp = o->hyai_delta[k] * o->ps0 + o->hybi_delta[k] * PSV(ie, o->np1)[p];
             o->Q[((size_t)ie * o->qsize + q) * E3 + p * L + k] = QDP(ie, o->np1_qdp, q)[p * L + k] / dp;
           }
+    if (diag) {
+      prim_diag_scalars(o, 0, 1);
+      prim_energy_halftimes(o, 0, 1);
+    }
   }
   {
     const int tmp = o->np1;
