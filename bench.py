@@ -187,6 +187,10 @@ def run_adjoint(rsplit, steps=10, traj_steps=48, every=8):
         adj = (time.perf_counter() - t0) / steps
         progress("adjoint: forward %.3f ms/step, adjoint %.3f ms/step" % (fwd * 1e3, adj * 1e3))
         tgt, w = synthetic_target(prob, 6), region_weight(prob)
+        # warm-up: a 2-step trajectory and its gradient (first-use costs of
+        # the loss's torch kernels and of the checkpoint allocations)
+        x = [a.clone().requires_grad_() for a in initial_state(prob)]
+        energy_mismatch(prob, *trajectory(s, *x, 2, 1), tgt, w).backward()
         x = [a.clone().requires_grad_() for a in initial_state(prob)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
