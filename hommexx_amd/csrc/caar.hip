@@ -16,12 +16,17 @@
 //   D: energy gradient, vorticity, v(np1)
 // With rsplit == 0 phases B and D also carry the Eulerian vertical advection
 // (eta_dot_dpdn from the column sums of div(v dp)).
+// XL (rsplit > 0, nlev <= CAAR_LDS_MAX_NLEV): div(v dp) and then the phi
+// terms live in one LDS array of the block's 4 columns x nlev, never in HBM
+// (phi is internal to the stage); otherwise both go through global scratch.
 // Outputs go to pre-exchange buffers; the DSS kernel writes the state.
 #include "kernels.hpp"
 
 namespace hx {
 
 constexpr int CNC = 4;  // level chunks per column
+// the LDS array of 4 columns x nlev levels fits the 64 KB a block may ask for
+constexpr int CAAR_LDS_MAX_NLEV = 128;
 
 __device__ __forceinline__ double virtual_t(const CaarArgs &a, const double *qv, int k, double t, double dp) {
   if (!qv) return t;
@@ -31,8 +36,9 @@ __device__ __forceinline__ double virtual_t(const CaarArgs &a, const double *qv,
   return t * qt;
 }
 
-template <bool RS0>  // rsplit == 0: Eulerian vertical advection
+template <bool RS0, bool XL>  // rsplit == 0: Eulerian vertical advection; XL: divv/phi in LDS
 __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
+  extern __shared__ double xs[];
   const int pt = threadIdx.x & 15;
   const int chunk = threadIdx.x >> 6;
   const int ie = blockIdx.x * 4 + ((threadIdx.x >> 4) & 3);
@@ -49,6 +55,9 @@ __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
   }
   const size_t sb = (size_t)(on ? ie : 0) * L * 16 + pt;  // scalar [ie][lev][pt]
   const size_t vb = (size_t)(on ? ie : 0) * L * 32 + pt;  // vector [ie][lev][c][pt]
+  // this column's div(v dp) / phi levels, stride 16
+  double *const divv = XL ? xs + (size_t)((threadIdx.x >> 4) & 3) * L * 16 + pt : a.divv + sb;
+  double *const phi = XL ? divv : a.phi + sb;
   const double eta = a.eta_ave_w, dt = a.dt;
   const double *qv = a.qdp_n0 ? a.qdp_n0 + (size_t)(on ? ie : 0) * a.qsize * L * 16 + pt : nullptr;
 
@@ -70,7 +79,7 @@ __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
       const double vdp0 = u * dp, vdp1 = w * dp;
       a.vn0[v] += eta * vdp0;
       a.vn0[v + 16] += eta * vdp1;
-      a.divv[s] = sph_div(e, d, vdp0, vdp1);
+      divv[(size_t)k * 16] = sph_div(e, d, vdp0, vdp1);
     }
   }
   __syncthreads();
@@ -79,7 +88,7 @@ __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
   // with S the serial column sums of div(v dp) (:264-301)
   double stotal = 0.0;
   if (on && RS0)
-    for (int k = 0; k < L; ++k) stotal += a.divv[sb + (size_t)k * 16];
+    for (int k = 0; k < L; ++k) stotal += divv[(size_t)k * 16];
   auto eta_of = [&](int k, double s_below) {  // s_below = S(k-1)
     return k == 0 ? 0.0 : (k < L ? a.hybi[k] * stotal - s_below : 0.0);
   };
@@ -98,9 +107,13 @@ __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
     const double facm = 0.5 * (1 / dpk) * ek;
     return facp * (x[(size_t)(k + 1) * stride] - xk) + facm * (xk - x[(size_t)(k - 1) * stride]);
   };
+  double integ = 0.0;
+  if (on)
+    for (int k = 0; k < k0; ++k) integ = integ + divv[(size_t)k * 16];
+  // XL: the phi terms below overwrite div(v dp) level by level, so every
+  // chunk first reads the levels above it
+  if (XL) __syncthreads();
   if (on) {
-    double integ = 0.0;
-    for (int k = 0; k < k0; ++k) integ = integ + a.divv[sb + (size_t)k * 16];
     double p_prev = p_top, dp_prev = dp_top;
     for (int k = k0; k < k1; ++k) {
       const size_t s = sb + (size_t)k * 16, v = vb + (size_t)k * 32;
@@ -111,19 +124,19 @@ __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
       const double p = p_prev + 0.5 * (dp_prev + dp);
       p_prev = p;
       dp_prev = dp;
-      const double divv = a.divv[s];
+      const double dvk = divv[(size_t)k * 16];
       double ek = 0.0, ek1 = 0.0, tva = 0.0;
       if (RS0) {
         ek = eta_of(k, integ);
-        ek1 = eta_of(k + 1, integ + divv);
+        ek1 = eta_of(k + 1, integ + dvk);
         tva = vadv(a.t_n0 + sb, 16, k, dp, ek, ek1);
         a.eta_acc[s] += eta * ek;
       }
       double gp0, gp1;
       sph_grad(e, d, p, gp0, gp1);
       const double vgrad_p = u * gp0 + w * gp1;
-      const double om = (vgrad_p - (integ + 0.5 * divv)) / p;
-      integ = integ + divv;
+      const double om = (vgrad_p - (integ + 0.5 * dvk)) / p;
+      integ = integ + dvk;
       a.omega[s] += eta * om;
       double gt0, gt1;
       sph_grad(e, d, t, gt0, gt1);
@@ -133,24 +146,23 @@ __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
       temp *= e.sph;
       a.out_t[s] = temp;
       double tmp = ek1;
-      tmp += divv;
+      tmp += dvk;
       tmp -= ek;
       tmp = a.dp_nm1[s] - tmp * dt;
       a.out_dp[s] = e.sph * tmp;
-      a.phi[s] = RGAS * tv * (dp * 0.5 / p);
+      phi[(size_t)k * 16] = RGAS * tv * (dp * 0.5 / p);
     }
   }
   __syncthreads();
   // ---------------------------------------------------------------- C
   double integration = 0.0;
   if (on)
-    for (int k = L - 1; k >= k1; --k) integration = integration + a.phi[sb + (size_t)k * 16];
+    for (int k = L - 1; k >= k1; --k) integration = integration + phi[(size_t)k * 16];
   __syncthreads();  // every chunk has read the terms below it before they are overwritten
   if (on)
     for (int k = k1 - 1; k >= k0; --k) {
-      const size_t s = sb + (size_t)k * 16;
-      const double r = a.phi[s];
-      a.phi[s] = e.phis + 2.0 * integration + r;
+      const double r = phi[(size_t)k * 16];
+      phi[(size_t)k * 16] = e.phis + 2.0 * integration + r;
       integration = integration + r;
     }
   // ---------------------------------------------------------------- D
@@ -158,7 +170,7 @@ __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
     double p_prev = p_top, dp_prev = dp_top;
     double integ = 0.0;
     if (RS0)
-      for (int k = 0; k < k0; ++k) integ = integ + a.divv[sb + (size_t)k * 16];
+      for (int k = 0; k < k0; ++k) integ = integ + divv[(size_t)k * 16];
     for (int k = k0; k < k1; ++k) {
       const size_t s = sb + (size_t)k * 16, v = vb + (size_t)k * 32;
       const double dp = a.dp_n0[s];
@@ -172,13 +184,13 @@ __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
       sph_grad(e, d, p, gp0, gp1);
       double eg0 = RGAS * (tv / p) * gp0;
       double eg1 = RGAS * (tv / p) * gp1;
-      const double ephi = 0.5 * (u * u + w * w) + a.phi[s];
+      const double ephi = 0.5 * (u * u + w * w) + phi[(size_t)k * 16];
       sph_grad_update(e, d, ephi, eg0, eg1);
       double vort = sph_vort(e, d, u, w);
       vort += e.fcor;
       double va0 = 0.0, va1 = 0.0;
       if (RS0) {
-        const double dv = a.divv[s];
+        const double dv = divv[(size_t)k * 16];
         const double ek = eta_of(k, integ), ek1 = eta_of(k + 1, integ + dv);
         integ = integ + dv;
         va0 = vadv(a.v_n0 + vb, 32, k, dp, ek, ek1);
@@ -200,8 +212,11 @@ __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
 
 void launch_caar(hipStream_t st, const CaarArgs &a) {
   const int blocks = (a.nelem + 3) / 4;
-  if (a.rsplit0) hipLaunchKernelGGL(caar_kernel<true>, dim3(blocks), dim3(64 * CNC), 0, st, a);
-  else hipLaunchKernelGGL(caar_kernel<false>, dim3(blocks), dim3(64 * CNC), 0, st, a);
+  if (a.rsplit0) hipLaunchKernelGGL((caar_kernel<true, false>), dim3(blocks), dim3(64 * CNC), 0, st, a);
+  else if (a.nlev <= CAAR_LDS_MAX_NLEV)
+    hipLaunchKernelGGL((caar_kernel<false, true>), dim3(blocks), dim3(64 * CNC), (size_t)4 * a.nlev * 16 * sizeof(double),
+                       st, a);
+  else hipLaunchKernelGGL((caar_kernel<false, false>), dim3(blocks), dim3(64 * CNC), 0, st, a);
 }
 
 // ------------------------------------------------------------------ small

@@ -30,19 +30,31 @@ P0 = 100000.0
 
 class VCoord:
     """Hybrid sigma-pressure coefficients (interfaces ``hyai/hybi``, midpoints
-    ``hyam/hybm``), read from the package's JSON tables."""
+    ``hyam/hybm``), read from the package's JSON tables (L26, L72).  Other
+    level counts (at least 3) get a synthetic coordinate: the 72-level
+    interfaces interpolated linearly in the interface index onto nlev + 1
+    points, midpoints as interface averages."""
 
     def __init__(self, nlev, ps0=P0):
-        path = os.path.join(os.path.dirname(__file__), "data", "vcoord_L%d.json" % nlev)
-        if not os.path.exists(path):
-            raise ValueError("no vertical coordinate table for nlev=%d" % nlev)
-        d = json.load(open(path))
+        data = os.path.join(os.path.dirname(__file__), "data")
+        path = os.path.join(data, "vcoord_L%d.json" % nlev)
         self.nlev = nlev
         self.ps0 = float(ps0)
-        self.hyai = np.array(d["hyai"], float)
-        self.hybi = np.array(d["hybi"], float)
-        self.hyam = np.array(d["hyam"], float)
-        self.hybm = np.array(d["hybm"], float)
+        if os.path.exists(path):
+            d = json.load(open(path))
+            self.hyai = np.array(d["hyai"], float)
+            self.hybi = np.array(d["hybi"], float)
+            self.hyam = np.array(d["hyam"], float)
+            self.hybm = np.array(d["hybm"], float)
+            return
+        if nlev < 3:
+            raise ValueError("no vertical coordinate for nlev=%d" % nlev)
+        d = json.load(open(os.path.join(data, "vcoord_L72.json")))
+        x = np.linspace(0.0, 72.0, nlev + 1)
+        self.hyai = np.interp(x, np.arange(73.0), np.array(d["hyai"], float))
+        self.hybi = np.interp(x, np.arange(73.0), np.array(d["hybi"], float))
+        self.hyam = 0.5 * (self.hyai[1:] + self.hyai[:-1])
+        self.hybm = 0.5 * (self.hybi[1:] + self.hybi[:-1])
 
     @property
     def etam(self):

@@ -126,6 +126,19 @@ def test_l72_parity():
         assert e <= TOL1, (k, e)
 
 
+@pytest.mark.parametrize("nlev", [160, 256])
+def test_many_levels_parity(nlev):
+    # above 128 levels CAAR keeps div(v dp) and phi in global scratch; 256 is
+    # the largest nlev (the remap's LDS then holds exactly 64 KB)
+    prob = build_problem(ne=2, nlev=nlev, qsize=2, seed=3)
+    print("case: ne2 L%d Q2, 2 steps" % nlev, flush=True)
+    out, ref = run_both(prob, 2)
+    rep = compare(out, ref)
+    print(fmt(rep))
+    for k, (e, _) in rep.items():
+        assert e <= TOL1 * 10, (k, e)
+
+
 def test_no_tracers():
     prob = build_problem(ne=4, nlev=26, qsize=0, seed=5)
     out, ref = run_both(prob, 2)

@@ -287,3 +287,19 @@ def test_isothermal_rest_state_stays_at_rest(oracle_lib):
     assert np.abs(out["ps_v"][:, n0] / 1e5 - 1).max() < 1e-13
     assert np.abs(out["Q"][:, 0] - Q).max() < 1e-13
     s.close()
+
+
+def test_synthetic_vertical_coordinate():
+    """Level counts without a table get the 72-level interfaces interpolated
+    in index: same top and bottom, strictly increasing pressure at p0,
+    midpoints between their interfaces."""
+    from hommexx_amd.ics import VCoord
+    ref = VCoord(72)
+    for nlev in (3, 100, 160, 256):
+        vc = VCoord(nlev)
+        assert len(vc.hyai) == nlev + 1 and len(vc.hyam) == nlev
+        assert vc.hyai[0] == ref.hyai[0] and vc.hybi[-1] == ref.hybi[-1] and vc.hybi[0] == ref.hybi[0]
+        pi = vc.hyai * vc.ps0 + vc.hybi * vc.ps0
+        assert np.all(np.diff(pi) > 0)
+        pm = vc.hyam * vc.ps0 + vc.hybm * vc.ps0
+        assert np.all((pm > pi[:-1]) & (pm < pi[1:]))
