@@ -51,46 +51,42 @@ __global__ __launch_bounds__(256) void hv_lap2_pre_kernel(HvArgs a) {
   const size_t s = ((size_t)ie * a.nlev + k) * 16 + pt;
   const size_t v = ((size_t)ie * a.nlev + k) * 32 + pt;
   // second laplacian: constant coefficient, or the tensor form
-  // (HyperviscosityFunctorImpl.hpp:88-128)
-  double tt, dpt, vt0, vt1;
-  if (TENSOR) {
-    double tv[4], vs[6];
-    load_tensor(a.m, ie, pt, tv, vs);
-    tt = sph_laplace_tensor(e, d, tv, a.tt_b[s]);
-    dpt = sph_laplace_tensor(e, d, tv, a.dpt_b[s]);
-    sph_vlaplace_cartesian(e, d, tv, vs, a.vt_b[v], a.vt_b[v + 16], vt0, vt1);
-  } else {
-    tt = sph_laplace(e, d, a.tt_b[s]);
-    dpt = sph_laplace(e, d, a.dpt_b[s]);
-    sph_vlaplace_contra(e, d, a.nu_ratio2, a.vt_b[v], a.vt_b[v + 16], vt0, vt1);
+  // (HyperviscosityFunctorImpl.hpp:88-128), then the pre-exchange scaling
+  // with the nu_top sponge on the top three levels; each field is finished
+  // and stored before the next one starts (fewer live registers)
+  double tv[4], vs[6];
+  if (TENSOR) load_tensor(a.m, ie, pt, tv, vs);
+  const bool top = a.nu_top > 0 && k < 3;  // uniform over the slab
+  const double sc = (k == 0 ? 4.0 : k == 1 ? 2.0 : 1.0) * a.nu_top;
+  {
+    double vt0, vt1;
+    if (TENSOR) sph_vlaplace_cartesian(e, d, tv, vs, a.vt_b[v], a.vt_b[v + 16], vt0, vt1);
+    else sph_vlaplace_contra(e, d, a.nu_ratio2, a.vt_b[v], a.vt_b[v + 16], vt0, vt1);
+    vt0 *= -a.nu;
+    vt1 *= -a.nu;
+    if (top) {
+      double lv0, lv1;
+      sph_vlaplace_contra(e, d, 1.0, a.v[v], a.v[v + 16], lv0, lv1);
+      vt0 += sc * lv0;
+      vt1 += sc * lv1;
+    }
+    a.vt_a[v] = vt0;
+    a.vt_a[v + 16] = vt1;
   }
-  // pre-exchange
+  {
+    double tt = TENSOR ? sph_laplace_tensor(e, d, tv, a.tt_b[s]) : sph_laplace(e, d, a.tt_b[s]);
+    tt *= -a.nu_s;
+    if (top) tt += sc * sph_laplace(e, d, a.t[s]);
+    a.tt_a[s] = tt;
+  }
   const double dpn = a.dp[s];
+  double dpt = TENSOR ? sph_laplace_tensor(e, d, tv, a.dpt_b[s]) : sph_laplace(e, d, a.dpt_b[s]);
   a.dpdiss_ave[s] += a.eta_ave_w * dpn / a.nsub;
   a.dpdiss_bih[s] += a.eta_ave_w * dpt / a.nsub;
-  double lv0 = 0, lv1 = 0, lt = 0, ldp = 0;
-  const bool top = a.nu_top > 0 && k < 3;
-  if (top) {  // uniform over the slab
-    sph_vlaplace_contra(e, d, 1.0, a.v[v], a.v[v + 16], lv0, lv1);
-    lt = sph_laplace(e, d, a.t[s]);
-    ldp = sph_laplace(e, d, dpn);
-  }
-  vt0 *= -a.nu;
-  vt1 *= -a.nu;
-  tt *= -a.nu_s;
   dpt *= -a.nu_p;
-  if (top) {
-    const double sc = (k == 0 ? 4.0 : k == 1 ? 2.0 : 1.0) * a.nu_top;
-    vt0 += sc * lv0;
-    vt1 += sc * lv1;
-    tt += sc * lt;
-    dpt += sc * ldp;
-  }
+  if (top) dpt += sc * sph_laplace(e, d, dpn);
   dpt *= a.dt;
   dpt += dpn * e.sph;
-  a.vt_a[v] = vt0;
-  a.vt_a[v + 16] = vt1;
-  a.tt_a[s] = tt;
   a.dpt_a[s] = dpt;
 }
 
