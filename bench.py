@@ -7,7 +7,7 @@ one prim_run_subcycle_c call through the C-ABI with all state resident in HBM.
 The same steps are then run in the mixed-precision form from the same state
 and its speed and per-field error against the FP64 run are reported, and the
 cost of an adjoint step against a forward step is measured on the ne16 L72
-Q4 differentiable-form workload (dry, rsplit = 0).
+Q4 differentiable-form workload (dry, rsplit = 1 and rsplit = 0).
 
 Prints one JSON line: SYPD = dt / (365 * seconds per step), the roofline of the
 dominant kernel (HIP-event durations on the library stream over the timed
@@ -16,6 +16,7 @@ fraction of HBM peak under the canonical bytes-per-step model, and a CPU
 baseline (the oracle, OpenMP over elements, one step of the same workload).
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-mixed] [--no-adjoint]
+                    [--ne 30] [--nlev 72] [--qsize 40] [--dt 300] [--nu 1e15] [--t-noise 0.0]
 """
 import argparse
 import json
@@ -99,8 +100,10 @@ def progress(msg):
 
 def make_problem(args):
     from hommexx_amd.problem import build_problem
-    return build_problem(ne=args.ne, nlev=args.nlev, qsize=args.qsize, seed=1, dt=args.dt, nu=1e15, nu_p=1e15,
-                         nu_q=1e15, nu_s=1e15, nu_div=1e15, nu_top=2.5e5, hypervis_subcycle=3, rsplit=1)
+    nu = args.nu
+    return build_problem(ne=args.ne, nlev=args.nlev, qsize=args.qsize, seed=1, dt=args.dt, nu=nu, nu_p=nu,
+                         nu_q=nu, nu_s=nu, nu_div=nu, nu_top=2.5e5, hypervis_subcycle=3, rsplit=1,
+                         t_noise=args.t_noise)
 
 
 def run_gpu(args, prob, form="fp64"):
@@ -232,6 +235,9 @@ def main():
     ap.add_argument("--nlev", type=int, default=72)
     ap.add_argument("--qsize", type=int, default=40)
     ap.add_argument("--dt", type=float, default=300.0)
+    ap.add_argument("--nu", type=float, default=1e15, help="hyperviscosity (all of nu, nu_p, nu_q, nu_s, nu_div)")
+    ap.add_argument("--t-noise", type=float, default=0.0,
+                    help="seeded white-noise temperature perturbation (K) on the initial state")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-cost leg")
@@ -253,6 +259,7 @@ def main():
         a1["rsplit0"] = run_adjoint(0)
         print(json.dumps({"adjoint": a1}))
         return
+    progress("building the ne%d L%d Q%d problem on the host" % (args.ne, args.nlev, args.qsize))
     prob = make_problem(args)
     sec, prof, dry_drift, q_drift, st64 = run_gpu(args, prob, "fp64")
     nelem, nlev, q = prob.nelem, prob.nlev, args.qsize
@@ -298,8 +305,9 @@ def main():
         "metric": METRIC, "value": round(sypd, 3), "unit": "SYPD", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "preqx step ne%d np4 L%d Q%d, dt=%gs, hv subcycle 3, remap every step, limiter 8, "
-                               "JW baroclinic" % (args.ne, nlev, q, args.dt),
+        "config": {"workload": "preqx step ne%d np4 L%d Q%d, dt=%gs, nu=%g, hv subcycle 3, remap every step, "
+                               "limiter 8, JW baroclinic%s" % (args.ne, nlev, q, args.dt, args.nu,
+                                                               " + %g K T noise" % args.t_noise if args.t_noise else ""),
                    "nelem": nelem, "nlev": nlev, "qsize": q, "form": "fp64"},
         "roofline": roof, "step_roofline": step_roof, "cpu_baseline": cpu,
         "conservation": {"dry_mass_rel_drift": dry_drift, "tracer_mass_rel_drift_max": q_drift},

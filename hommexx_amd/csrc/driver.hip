@@ -81,7 +81,9 @@ struct Ctx {
   double *cv = nullptr, *ct = nullptr, *cdp = nullptr, *cdiv = nullptr;
   double *vt_a = nullptr, *tt_a = nullptr, *dpt_a = nullptr, *vt_b = nullptr, *tt_b = nullptr, *dpt_b = nullptr;
   double *qdp_pre = nullptr, *qbih = nullptr, *qbih_pre = nullptr, *qlim[2] = {}, *fdss_pre = nullptr;
-  double *qdp_pre2 = nullptr, *fdss_pre2 = nullptr;  // stage-2 pre-exchange buffers
+  // stage-2 pre-exchange buffers: aliases of qbih_pre and ct, which are free
+  // from the stage-2 advection until its exchange has consumed them
+  double *qdp_pre2 = nullptr, *fdss_pre2 = nullptr;
   int *err_d = nullptr;
   int *err_h = nullptr;
   // diagnostics: the Fortran arrays of init_diagnostics_c (q, qvar, qmass,
@@ -310,13 +312,13 @@ void alloc_work() {
   c.dpt_a = dalloc<double>(e3);
   c.dpt_b = dalloc<double>(e3);
   c.qdp_pre = dalloc<double>(q3);
-  c.qdp_pre2 = dalloc<double>(q3);
   c.qbih = dalloc<double>(q3);
   c.qbih_pre = dalloc<double>(q3);
+  c.qdp_pre2 = c.qbih_pre;
   c.qlim[0] = dalloc<double>((size_t)c.nelem * (c.qsize ? c.qsize : 1) * 2 * c.nlev);
   c.qlim[1] = dalloc<double>((size_t)c.nelem * (c.qsize ? c.qsize : 1) * 2 * c.nlev);
   c.fdss_pre = dalloc<double>(e3);
-  c.fdss_pre2 = dalloc<double>(e3);
+  c.fdss_pre2 = c.ct;
   c.err_d = dalloc<int>(1);
   if (!c.err_h) hipHostMalloc((void **)&c.err_h, sizeof(int));
   c.be_set = true;
@@ -820,6 +822,18 @@ void push_tracers(double *f90, const double *dev, int ntl_f, int tl) {
              e3 * sizeof(double));
 }
 
+// CAM forcing buffers FT, FM, FQ, allocated (zeroed, as the reference's
+// views are) the first time a forcing run or a forcing push needs them
+bool ensure_forcing() {
+  Ctx &c = C();
+  if (c.ft) return true;
+  const size_t e3 = (size_t)c.nelem * c.nlev * 16;
+  c.ft = dalloc<double>(e3);
+  c.fm = dalloc<double>(2 * e3);
+  c.fq = dalloc<double>(e3 * (c.qsize ? c.qsize : 1));
+  return c.ft && c.fm && c.fq;
+}
+
 // ------------------------------------------------------------ diagnostics
 // Diagnostics.cpp:37-185.  The device sums land in a staging buffer; slot
 // ivar of each Fortran array is written with strided copies (the other slots
@@ -1158,9 +1172,7 @@ void init_elements_2d_c(const int &num_elems, const double *const &D, const doub
   c.divdp = dalloc<double>(e3);
   c.divdp_proj = dalloc<double>(e3);
   c.vn0 = dalloc<double>(2 * e3);
-  c.ft = dalloc<double>(e3);
-  c.fm = dalloc<double>(2 * e3);
-  c.fq = dalloc<double>(e3 * (c.qsize ? c.qsize : 1));
+  c.ft = c.fm = c.fq = nullptr;  // CAM forcing buffers: on first use (ensure_forcing)
   c.elems_set = true;
 }
 
@@ -1250,6 +1262,7 @@ void prim_run_subcycle_c(const double &dt, int &nstep, int &nm1, int &n0, int &n
     diag_scalars(true, 3);
     diag_energy(true, 2);
   }
+  if ((c.p.ftype == 0 || c.p.ftype == 2) && !ensure_forcing()) return;
   if (c.p.ftype == 0) {
     Scope s("forcing");
     launch_state_forcing(c.st, c.nelem, c.nlev, dt_remap, c.ft, c.fm, c.t[c.n0], c.v[c.n0]);
@@ -1328,6 +1341,7 @@ void cxx_push_results_to_f90(double *&v, double *&T, double *&dp3d, double *&Qdp
 // cxx_f90_interface.cpp:157-178
 void cxx_push_forcing_to_f90(double *FM, double *FT, double *FQ) {
   Ctx &c = C();
+  if (!ensure_forcing()) return;
   hipStreamSynchronize(c.st);
   const size_t e3 = (size_t)c.nlev * 16;
   push_levels(FM, c.fm, 1, 0, 2 * e3);
@@ -1338,6 +1352,7 @@ void cxx_push_forcing_to_f90(double *FM, double *FT, double *FQ) {
 // cxx_f90_interface.cpp:180-205
 void f90_push_forcing_to_cxx(double *FM, double *FT, double *FQ, double *Qdp) {
   Ctx &c = C();
+  if (!ensure_forcing()) return;
   const size_t e3 = (size_t)c.nlev * 16;
   hipStreamSynchronize(c.st);
   pull_levels(c.fm, FM, 1, 0, 2 * e3);
