@@ -10,11 +10,13 @@ PPM remap) over the
 segment's steps in reverse.  Device memory: ``ceil(nsteps/every)``
 checkpoints plus ``every`` step states, each (v, T, ps).
 
-The adjoint covers dry configurations with ``rsplit`` 0 or 1.  With
-``rsplit = 1`` the states are remapped every step and the backward pass runs
-the linearised PPM remap (branches frozen from the forward pass); with
-``rsplit = 0`` the remap moves only tracers.  Tracers do not act on the
-dynamics of a dry run, so they have no adjoint.
+The adjoint covers dry configurations with any ``rsplit`` and ``qsplit``: one
+``prim_run_subcycle_c`` call is qsplit * max(rsplit, 1) dynamics steps (RK5
+stages and hyperviscosity) followed by the remap.  With ``rsplit > 0`` the
+states are remapped and the backward pass runs the linearised PPM remap
+(branches frozen from the forward pass); with ``rsplit = 0`` the remap moves
+only tracers.  Tracers do not act on the dynamics of a dry run, so they have
+no adjoint.
 
 State tensors use the library's device layouts: v ``(nelem, nlev, 2, 4, 4)``,
 T ``(nelem, nlev, 4, 4)``, ps ``(nelem, 4, 4)``, float64 on the GPU.
@@ -38,12 +40,13 @@ def _ptr(t):
 
 class DiffSession:
     """A :class:`HipSession` with device-side state access and the adjoint
-    step.  ``prob.params`` must describe a dry run with ``rsplit`` 0 or 1."""
+    step.  ``prob.params`` must describe a dry run with constant-coefficient
+    hyperviscosity and no forcing."""
 
     def __init__(self, prob, device="cuda"):
         p = prob.params
-        if p.rsplit > 1 or p.moisture or p.qsplit != 1 or p.hypervis_scaling != 0.0 or p.ftype != -1:
-            raise HommexxError("the adjoint needs rsplit=0 or 1, qsplit=1, no moisture, constant HV and no forcing")
+        if p.moisture or p.hypervis_scaling != 0.0 or p.ftype != -1:
+            raise HommexxError("the adjoint needs no moisture, constant HV and no forcing")
         self.prob = prob
         self.device = torch.device(device)
         self.h = HipSession(prob)

@@ -621,7 +621,9 @@ void remap(double dt_remap) {
 struct AdjBufs {
   size_t f3 = 0;  // doubles in one 3-D scalar
   int nsub = 0;
+  int nd = 0;  // dynamics steps per prim_run_subcycle_c
   std::vector<StateBuf> U, H;  // CAAR stage inputs U0..U4, HV cycle states H0..Hnsub
+  std::vector<StateBuf> X;     // input states of the nd dynamics steps
   StateBuf W{}, LX{}, L0{}, L1{}, LP{}, S{}, M{};
   double *div, *p, *cc, *stot, *pb, *phib, *etab, *ombp, *divb, *lv, *lt, *ldp;
   double *rscr = nullptr;  // remap adjoint scratch (rsplit > 0)
@@ -640,23 +642,26 @@ StateBuf carve_state(double *&q, size_t f3) {
   return s;
 }
 
-bool adj_alloc() {
+bool adj_alloc(int nd) {
   Ctx &c = C();
   const size_t f3 = (size_t)c.nelem * c.nlev * 16;
   const int nsub = c.p.hypervis_subcycle;
-  if (adjb && adjb->f3 == f3 && adjb->nsub == nsub) return true;
+  if (adjb && adjb->f3 == f3 && adjb->nsub == nsub && adjb->nd == nd) return true;
   if (!adjb) adjb = new AdjBufs();
   AdjBufs &b = *adjb;
-  const int nstates = 5 + 1 + (nsub + 1) + 6;  // U, W, H, LX L0 L1 LP S M
+  const int nstates = 5 + 1 + (nsub + 1) + 6 + nd;  // U, W, H, LX L0 L1 LP S M, X
   const size_t nrs = adj_remap_scratch(c.nelem, c.nlev);
   const size_t total = (size_t)nstates * 4 * f3 + 12 * f3 + (size_t)c.nelem * 16 + nrs;
   b.base = dalloc<double>(total);  // owned by the session (freed with it)
   if (!b.base) return false;
   b.f3 = f3;
   b.nsub = nsub;
+  b.nd = nd;
   double *q = b.base;
   b.U.clear();
   b.H.clear();
+  b.X.clear();
+  for (int i = 0; i < nd; ++i) b.X.push_back(carve_state(q, f3));
   for (int i = 0; i < 5; ++i) b.U.push_back(carve_state(q, f3));
   b.W = carve_state(q, f3);
   for (int i = 0; i <= nsub; ++i) b.H.push_back(carve_state(q, f3));
@@ -1476,23 +1481,25 @@ int hommexx_hip_adjoint_step(double dt, const double *v, const double *T, const 
                              double *lam_T, double *lam_ps) {
   Ctx &c = C();
   if (!ready("hommexx_hip_adjoint_step")) return 1;
-  if (c.p.rsplit > 1 || c.p.qsplit != 1 || c.p.moist || !c.consthv || c.form != 0 || c.p.ftype != -1) {
-    fail(70, "hommexx_hip_adjoint_step: needs rsplit = 0 or 1, qsplit = 1, no moisture, constant-coefficient "
-             "hyperviscosity, no forcing and the FP64 form");
+  if (c.p.moist || !c.consthv || c.form != 0 || c.p.ftype != -1) {
+    fail(70, "hommexx_hip_adjoint_step: needs a dry run, constant-coefficient hyperviscosity, no forcing and the "
+             "FP64 form");
     return 2;
   }
   if (!c.be_set) init_boundary_exchanges_c();
-  if (!adj_alloc()) return 3;
+  // prim_run_subcycle_c: qsplit dynamics steps per prim_step, rsplit (at
+  // least one) prim_steps, then the remap (prim_driver.cpp:115-130,
+  // prim_step.cpp:57-80)
+  const int nd = c.p.qsplit * (c.p.rsplit > 0 ? c.p.rsplit : 1);
+  if (!adj_alloc(nd)) return 3;
   AdjBufs &b = *adjb;
   const size_t f3 = b.f3;
   const int nsub = c.p.hypervis_subcycle;
-  std::vector<StateBuf> &U = b.U, &H = b.H;
-  // forward: stage inputs and hyperviscosity states
-  {
-    Scope s("adj_forward");
-    hipMemcpyAsync(U[0].v, v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
-    hipMemcpyAsync(U[0].t, T, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
-    launch_dp3d_from_ps(c.st, c.nelem, c.nlev, c.hyai_d, c.hybi_d, c.ps0, ps, U[0].dp);
+  std::vector<StateBuf> &U = b.U, &H = b.H, &X = b.X;
+  // one dynamics step from X[i]: RK5 stage inputs U0..U4, w and the HV cycle
+  // states H0..Hnsub (H[nsub] is the step's output)
+  auto dyn_forward = [&](int i) {
+    copy_state(U[0], X[i], f3);
     caar_run(U[0], U[0], U[1], dt / 5.0, 0.0);
     caar_run(U[0], U[1], U[2], dt / 5.0, 0.0);
     caar_run(U[0], U[2], U[3], dt / 3.0, 0.0);
@@ -1504,6 +1511,18 @@ int hommexx_hip_adjoint_step(double dt, const double *v, const double *T, const 
       copy_state(H[cyc + 1], H[cyc], f3);
       launch_hv_cycle(c.st, hv_args(H[cyc + 1], dt, 0.0));
     }
+  };
+  // forward: the input of every dynamics step; the last step's stage states
+  // stay in U, W, H for the first reverse step
+  {
+    Scope s("adj_forward");
+    hipMemcpyAsync(X[0].v, v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+    hipMemcpyAsync(X[0].t, T, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+    launch_dp3d_from_ps(c.st, c.nelem, c.nlev, c.hyai_d, c.hybi_d, c.ps0, ps, X[0].dp);
+    for (int i = 0; i < nd; ++i) {
+      dyn_forward(i);
+      if (i + 1 < nd) copy_state(X[i + 1], H[nsub], f3);
+    }
   }
   // reverse sweep
   Scope s("adj_reverse");
@@ -1512,7 +1531,7 @@ int hommexx_hip_adjoint_step(double dt, const double *v, const double *T, const 
   if (c.p.rsplit == 0) {
     launch_adj_fill_dp(c.st, c.nelem, c.nlev, lam_ps, b.LX.dp);  // ps = hyai(0) ps0 + sum_k dp3d
   } else {
-    // linearised PPM remap of v and T from the thicknesses after hyperviscosity
+    // linearised PPM remap of v and T from the thicknesses of the last step
     AdjRemapArgs r{};
     r.nelem = c.nelem;
     r.nlev = c.nlev;
@@ -1530,15 +1549,21 @@ int hommexx_hip_adjoint_step(double dt, const double *v, const double *T, const 
     r.scr = b.rscr;
     launch_adj_remap(c.st, r);
   }
-  for (int cyc = nsub - 1; cyc >= 0; --cyc) hv_adjoint(H[cyc], H[cyc + 1], b.LX, dt / nsub);
-  launch_adj_zero(c.st, 4 * f3, b.L0.v);
-  launch_adj_zero(c.st, 4 * f3, b.L1.v);
-  // stage 5 from w = (5 u1 - u0)/4 and u4; then stages 4..1
-  caar_adjoint(U[4], b.LX, 3.0 * dt / 4.0, b.LX, false, b.L1, 1.25, &b.L0, -0.25);
-  caar_adjoint(U[3], b.LX, 2.0 * dt / 3.0, b.LX, false, b.L0, 1.0, nullptr, 0.0);
-  caar_adjoint(U[2], b.LX, dt / 3.0, b.LX, false, b.L0, 1.0, nullptr, 0.0);
-  caar_adjoint(U[1], b.LX, dt / 5.0, b.L1, true, b.L0, 1.0, nullptr, 0.0);
-  caar_adjoint(U[0], b.L1, dt / 5.0, b.L0, true, b.L0, 1.0, nullptr, 0.0);
+  for (int i = nd - 1; i >= 0; --i) {
+    if (i < nd - 1) {
+      dyn_forward(i);  // this step's stage states
+      copy_state(b.LX, b.L0, f3);  // adjoint of its output = adjoint of the next step's input
+    }
+    for (int cyc = nsub - 1; cyc >= 0; --cyc) hv_adjoint(H[cyc], H[cyc + 1], b.LX, dt / nsub);
+    launch_adj_zero(c.st, 4 * f3, b.L0.v);
+    launch_adj_zero(c.st, 4 * f3, b.L1.v);
+    // stage 5 from w = (5 u1 - u0)/4 and u4; then stages 4..1
+    caar_adjoint(U[4], b.LX, 3.0 * dt / 4.0, b.LX, false, b.L1, 1.25, &b.L0, -0.25);
+    caar_adjoint(U[3], b.LX, 2.0 * dt / 3.0, b.LX, false, b.L0, 1.0, nullptr, 0.0);
+    caar_adjoint(U[2], b.LX, dt / 3.0, b.LX, false, b.L0, 1.0, nullptr, 0.0);
+    caar_adjoint(U[1], b.LX, dt / 5.0, b.L1, true, b.L0, 1.0, nullptr, 0.0);
+    caar_adjoint(U[0], b.L1, dt / 5.0, b.L0, true, b.L0, 1.0, nullptr, 0.0);
+  }
   hipMemcpyAsync(lam_v, b.L0.v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
   hipMemcpyAsync(lam_T, b.L0.t, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
   launch_adj_ps(c.st, c.nelem, c.nlev, c.hybi_d, b.L0.dp, lam_ps);  // dp3d = dhyai ps0 + dhybi ps

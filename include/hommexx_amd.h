@@ -143,9 +143,10 @@ void hommexx_hip_time_levels(int *nm1, int *n0, int *np1, int *nstep, int *n0_qd
  * (set also recomputes dp3d(n0) from ps).  hommexx_hip_adjoint_step takes
  * the input state of one prim_run_subcycle_c step of length dt and, in
  * place, maps the adjoint of the step's output (v, T, ps) to the adjoint of
- * its input; it needs a dry configuration with rsplit 0 or 1, qsplit = 1,
- * constant-coefficient hyperviscosity, no forcing and the FP64 form, and
- * leaves the session's model state unchanged.  Return 0 on success. */
+ * its input (all qsplit * max(rsplit, 1) dynamics steps and the remap); it
+ * needs a dry configuration, constant-coefficient hyperviscosity, no forcing
+ * and the FP64 form, and leaves the session's model state unchanged.
+ * Return 0 on success. */
 int hommexx_hip_get_state(double *v, double *T, double *ps);
 int hommexx_hip_set_state(const double *v, const double *T, const double *ps);
 int hommexx_hip_adjoint_step(double dt, const double *v, const double *T, const double *ps, double *lam_v,

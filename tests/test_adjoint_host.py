@@ -10,8 +10,7 @@ from hommexx_amd.hip import HommexxError
 from hommexx_amd.problem import build_problem
 
 
-@pytest.mark.parametrize("over", [dict(moisture=True), dict(rsplit=2), dict(qsplit=2), dict(ftype=0),
-                                  dict(hypervis_scaling=3.2)])
+@pytest.mark.parametrize("over", [dict(moisture=True), dict(ftype=0), dict(hypervis_scaling=3.2)])
 def test_unsupported_configurations_are_rejected(over):
     prob = build_problem(ne=2, nlev=26, qsize=1, **over)
     with pytest.raises(HommexxError):

@@ -7,9 +7,10 @@
   of 1e-6, along 64 seeded directions at ne16 L72 Q4 over 48 steps with a
   checkpoint every 8 steps.
 
-Both dry configurations are covered: rsplit = 0 (Eulerian vertical
-coordinate) and rsplit = 1 (states remapped every step: the backward pass
-runs the linearised PPM remap with the forward pass's branches frozen).
+Dry configurations: rsplit = 0 (Eulerian vertical coordinate), rsplit = 1
+(states remapped every step: the backward pass runs the linearised PPM remap
+with the forward pass's branches frozen), rsplit = 3 (three dynamics steps
+per remap) and qsplit = 2.
 
 The gradient is taken of a regional kinetic-energy-weighted mismatch against
 a seeded synthetic target, with respect to the initial T and ps (and v).
@@ -34,11 +35,12 @@ def ne16_problem(rsplit):
                          nu_div=nu)
 
 
-@pytest.mark.parametrize("rsplit,alg", [(0, 1), (1, 1), (1, 2)])
-def test_adjoint_matches_torch_reference_ne4(rsplit, alg):
-    prob = build_problem(ne=4, nlev=26, qsize=2, seed=31, rsplit=rsplit, remap_alg=alg, t_noise=0.1)
+@pytest.mark.parametrize("rsplit,alg,qsplit", [(0, 1, 1), (1, 1, 1), (1, 2, 1), (3, 1, 1), (1, 1, 2), (0, 1, 2)])
+def test_adjoint_matches_torch_reference_ne4(rsplit, alg, qsplit):
+    prob = build_problem(ne=4, nlev=26, qsize=2, seed=31, rsplit=rsplit, remap_alg=alg, t_noise=0.1, qsplit=qsplit)
     nsteps, every = 6, 4
-    print("case: ne4 L26 rsplit %d alg %d, %d steps, checkpoint every %d" % (rsplit, alg, nsteps, every), flush=True)
+    print("case: ne4 L26 rsplit %d alg %d qsplit %d, %d steps, checkpoint every %d" % (rsplit, alg, qsplit, nsteps,
+                                                                                       every), flush=True)
     # pure-PyTorch FP64 reference on the CPU
     x0 = [a.clone().requires_grad_() for a in initial_state(prob, "cpu")]
     out = TorchStep(prob).run(*x0, nsteps)
@@ -57,7 +59,7 @@ def test_adjoint_matches_torch_reference_ne4(rsplit, alg):
         assert e <= 1e-9, (n, e)
 
 
-@pytest.mark.parametrize("rsplit", [0, 1])
+@pytest.mark.parametrize("rsplit", [0, 1, 3])
 def test_adjoint_checkpoint_interval_independent(rsplit):
     """The gradient does not depend on the checkpoint interval."""
     prob = build_problem(ne=4, nlev=26, qsize=0, seed=33, rsplit=rsplit)
@@ -73,10 +75,11 @@ def test_adjoint_checkpoint_interval_independent(rsplit):
             assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize("rsplit", [0, 1])
+@pytest.mark.parametrize("rsplit", [0, 1, 3])
 def test_adjoint_central_differences_ne16(rsplit):
     prob = ne16_problem(rsplit)
-    nsteps, every, ndir, eps = 48, 8, 64, 1e-6
+    # 48 dynamics steps: with rsplit 3 that is 16 remap intervals, checkpointed every 4
+    nsteps, every, ndir, eps = (16, 4, 64, 1e-6) if rsplit == 3 else (48, 8, 64, 1e-6)
     print("case: ne16 L72 Q4 rsplit %d, %d steps, checkpoint every %d, %d directions" % (rsplit, nsteps, every, ndir),
           flush=True)
     tgt, w = synthetic_target(prob, 6), region_weight(prob)

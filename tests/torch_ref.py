@@ -1,15 +1,18 @@
-"""Pure-PyTorch FP64 restatement of the dry preqx dynamics step with
-rsplit = 0 (Eulerian vertical coordinate), used as the reference gradient
-for the HIP adjoint through ``torch.autograd``.
+"""Pure-PyTorch FP64 restatement of the dry preqx step (one
+prim_run_subcycle_c: qsplit * max(rsplit, 1) dynamics steps, then the
+remap), used as the reference gradient for the HIP adjoint through
+``torch.autograd``.
 
 TEST INFRASTRUCTURE ONLY.  It restates, in vectorised torch, the same step
 the CPU oracle runs (``oracle/preqx_oracle.c``) for a configuration without
 moisture: dp3d from ps (prim_driver.cpp:101-110), the five CAAR stages of
 u3_5stage with their DSS (prim_advance_exp.cpp:113-161,
 CaarFunctorImpl.hpp:98-729), the subcycled biharmonic hyperviscosity with
-the nu_top sponge (HyperviscosityFunctorImpl.cpp, biharmonic_wk_dp3d) and
-the rsplit = 0 surface-pressure update of the remap
-(PpmRemap.hpp / VerticalRemapManager.cpp: ps_v = hyai(1) ps0 + sum dp3d).
+the nu_top sponge (HyperviscosityFunctorImpl.cpp, biharmonic_wk_dp3d), the
+dynamics steps chained through dp3d (prim_step.cpp:57-80,
+prim_driver.cpp:115-130), and the remap: for rsplit = 0 the surface-pressure
+update ps_v = hyai(1) ps0 + sum dp3d, for rsplit > 0 the PPM remap of v and
+T (PpmRemap.hpp / VerticalRemapManager.cpp).
 Tracers do not feed back on the dynamics of a dry run, so they are left out.
 
 Fields use the device layout of the library: scalars ``(nelem, nlev, 4, 4)``
@@ -26,14 +29,14 @@ RRE = 1.0 / 6.376e6
 
 
 class TorchStep:
-    """One dry step (rsplit = 0 or 1) as a differentiable torch function."""
+    """One dry remap interval as a differentiable torch function."""
 
     def __init__(self, prob, device="cpu"):
         p = prob.params
-        if p.rsplit > 1 or p.moisture or p.hypervis_scaling != 0.0 or p.qsplit != 1 or p.remap_alg not in (1, 2):
-            raise ValueError("the torch reference covers dry rsplit=0/1, qsplit=1, constant-coefficient HV, "
-                             "remap alg 1/2")
+        if p.moisture or p.hypervis_scaling != 0.0 or p.remap_alg not in (1, 2):
+            raise ValueError("the torch reference covers dry runs with constant-coefficient HV and remap alg 1/2")
         self.rsplit = p.rsplit
+        self.ndyn = p.qsplit * max(p.rsplit, 1)
         self.remap_alg = p.remap_alg
         t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=device)
         m = prob.metric
@@ -221,10 +224,10 @@ class TorchStep:
         return torch.stack([n0, n1], 2), Tn, dpt * r
 
     # ----------------------------------------------------------------- step
-    def step(self, v, T, ps):
+    def dyn(self, u0):
+        """one dynamics step: RK5 stages (u3_5stage) and the hyperviscosity
+        subcycles, (v, T, dp3d) -> (v, T, dp3d)"""
         dt = self.dt
-        dp = self.dA + self.dB * ps.unsqueeze(1)
-        u0 = (v, T, dp)
         u1 = self.stage(u0, u0, dt / 5.0)
         u2 = self.stage(u0, u1, dt / 5.0)
         u3 = self.stage(u0, u2, dt / 3.0)
@@ -233,6 +236,12 @@ class TorchStep:
         x = self.stage(w, u4, 3.0 * dt / 4.0)
         for _ in range(self.nsub):
             x = self.hv_cycle(x, dt / self.nsub)
+        return x
+
+    def step(self, v, T, ps):
+        x = (v, T, self.dA + self.dB * ps.unsqueeze(1))
+        for _ in range(self.ndyn):
+            x = self.dyn(x)
         vn, Tn, dpn = x
         if self.rsplit == 0:
             return vn, Tn, self.ptop + dpn.sum(1)
