@@ -9,8 +9,9 @@
 // four-cell window suffices, and the target interface k+1 lies in source cell
 // kid[k] >= k-1 (kid is found by the same forward search).  Every value is
 // computed with the reference's expression and operand order; grid factors
-// that depend on the thicknesses only (the reference's ppmdx rows) are
-// evaluated once per column and shared by a batch of fields.
+// that depend on the thicknesses only (the reference's ppmdx rows) and the
+// target table are evaluated once per column and shared through LDS by all
+// the fields of the element's block.
 #include "kernels.hpp"
 
 namespace hx {
@@ -56,19 +57,8 @@ __device__ __forceinline__ double integrate_parabola(double sq, double lin, doub
   return (cst * (x2 - x1) + lin * (x2 * x2 - x1 * x1) / 2.0) + sq * (x2 * x2 * x2 - x1 * x1 * x1) / 3.0;
 }
 
-// dma(j) from a(j-1..j+1), dx(j-1..j+1)
-__device__ __forceinline__ double ppm_dma(double am1, double a0, double a1, double xm1, double x0, double x1) {
-  if ((a1 - a0) * (a0 - am1) > 0.0) {
-    const double g0 = x0 / (xm1 + x0 + x1);
-    const double g1 = (2.0 * xm1 + x0) / (x0 + x1);
-    const double g2 = (x0 + 2.0 * x1) / (xm1 + x0);
-    const double da = g0 * (g1 * (a1 - a0) + g2 * (a0 - am1));
-    return dmin3(fabs(da), 2.0 * fabs(a0 - am1), 2.0 * fabs(a1 - a0)) * copysign(1.0, da);
-  }
-  return 0.0;
-}
-
-// the same with the grid factors g0, g1, g2 precomputed
+// dma(j) from a(j-1..j+1) and the grid factors g0, g1, g2 of dx(j-1..j+1)
+// (the reference's ppmdx rows 1-3)
 __device__ __forceinline__ double ppm_dma_g(double am1, double a0, double a1, double g0, double g1, double g2) {
   if ((a1 - a0) * (a0 - am1) > 0.0) {
     const double da = g0 * (g1 * (a1 - a0) + g2 * (a0 - am1));
