@@ -11,56 +11,62 @@
 namespace hx {
 
 // One lane = one (element, level, point) for every field of the call: the
-// source table is read once and all the fields' gathers are issued before
-// any store, so each lane has up to 4*nf independent loads in flight.
+// source table is read once and all the fields' loads are issued before the
+// first sum, so each lane has up to 4*nf independent loads in flight.
 __global__ __launch_bounds__(256) void dss_kernel(DssArgs a) {
   int ie, k, pt;
   if (!group_slab(a.nelem, a.nlev, ie, k, pt)) return;
   const int *src = a.src + ((size_t)ie * 16 + pt) * 3;
   const int s0 = src[0], s1 = src[1], s2 = src[2];
-  const double r = a.rsmp ? a.rsmp[(size_t)ie * 16 + pt] : 1.0;
-  double x[DSS_MAX_FIELDS];
+  const double r0 = a.rsmp ? a.rsmp[(size_t)ie * 16 + pt] : 1.0;
+  // all loads of all fields are issued before the first sum (gather_sum);
+  // an absent source re-reads the own value
+  auto at = [&](const DssField &F, int src, size_t own) {
+    return opaque_index(src >= 0 ? (size_t)(src >> 4) * F.estride + (size_t)k * F.lstride + (src & 15) : own);
+  };
   if (a.f32) {
     // FP32 fields of the mixed form, accumulated in FP64
+    float r[DSS_MAX_FIELDS][4];
 #pragma unroll
     for (int f = 0; f < DSS_MAX_FIELDS; ++f) {
       if (f < a.nf) {
         const DssField &F = a.f[f];
         const float *in = (const float *)F.in;
-        const size_t off = (size_t)k * F.lstride;
-        double y = in[(size_t)ie * F.estride + off + pt];
-        if (s0 >= 0) y += (double)in[(size_t)(s0 >> 4) * F.estride + off + (s0 & 15)];
-        if (s1 >= 0) y += (double)in[(size_t)(s1 >> 4) * F.estride + off + (s1 & 15)];
-        if (s2 >= 0) y += (double)in[(size_t)(s2 >> 4) * F.estride + off + (s2 & 15)];
-        x[f] = y;
+        const size_t own = (size_t)ie * F.estride + (size_t)k * F.lstride + pt;
+        r[f][0] = in[own];
+        r[f][1] = in[at(F, s0, own)];
+        r[f][2] = in[at(F, s1, own)];
+        r[f][3] = in[at(F, s2, own)];
       }
     }
 #pragma unroll
     for (int f = 0; f < DSS_MAX_FIELDS; ++f) {
       if (f < a.nf) {
         const DssField &F = a.f[f];
-        ((float *)F.out)[(size_t)ie * F.estride + (size_t)k * F.lstride + pt] = (float)(a.rsmp ? x[f] * r : x[f]);
+        const double x = gather_sum(r[f], s0, s1, s2);
+        ((float *)F.out)[(size_t)ie * F.estride + (size_t)k * F.lstride + pt] = (float)(a.rsmp ? x * r0 : x);
       }
     }
     return;
   }
+  double r[DSS_MAX_FIELDS][4];
 #pragma unroll
   for (int f = 0; f < DSS_MAX_FIELDS; ++f) {
     if (f < a.nf) {
       const DssField &F = a.f[f];
-      const size_t off = (size_t)k * F.lstride;
-      double y = F.in[(size_t)ie * F.estride + off + pt];
-      if (s0 >= 0) y += F.in[(size_t)(s0 >> 4) * F.estride + off + (s0 & 15)];
-      if (s1 >= 0) y += F.in[(size_t)(s1 >> 4) * F.estride + off + (s1 & 15)];
-      if (s2 >= 0) y += F.in[(size_t)(s2 >> 4) * F.estride + off + (s2 & 15)];
-      x[f] = y;
+      const size_t own = (size_t)ie * F.estride + (size_t)k * F.lstride + pt;
+      r[f][0] = F.in[own];
+      r[f][1] = F.in[at(F, s0, own)];
+      r[f][2] = F.in[at(F, s1, own)];
+      r[f][3] = F.in[at(F, s2, own)];
     }
   }
 #pragma unroll
   for (int f = 0; f < DSS_MAX_FIELDS; ++f) {
     if (f < a.nf) {
       const DssField &F = a.f[f];
-      F.out[(size_t)ie * F.estride + (size_t)k * F.lstride + pt] = a.rsmp ? x[f] * r : x[f];
+      const double x = gather_sum(r[f], s0, s1, s2);
+      F.out[(size_t)ie * F.estride + (size_t)k * F.lstride + pt] = a.rsmp ? x * r0 : x;
     }
   }
 }

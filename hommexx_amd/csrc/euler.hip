@@ -83,10 +83,12 @@ __global__ __launch_bounds__(256) void euler_nbr_minmax_kernel(EulerArgs a) {
   const int q = (int)(i / ((size_t)L * a.nelem));
   const size_t own = (((size_t)ie * Q + q) * 2) * L + k;
   double mn = a.qlim[own], mx = a.qlim[own + L];
+  // an absent neighbour (-1) reads the element's own row, which leaves the
+  // min and max unchanged: no branch, all loads in flight together
+#pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int nb = a.nbr[ie * 8 + c];
-    if (nb < 0) continue;
-    const size_t o = (((size_t)nb * Q + q) * 2) * L + k;
+    const size_t o = opaque_index((((size_t)(nb < 0 ? ie : nb) * Q + q) * 2) * L + k);
     mx = dmax(mx, a.qlim[o + L]);
     mn = dmin(mn, a.qlim[o]);
   }
@@ -95,16 +97,26 @@ __global__ __launch_bounds__(256) void euler_nbr_minmax_kernel(EulerArgs a) {
 }
 
 // gather of point (ie, pt) at level k from a levels-fastest field with
-// element stride es (own value, then the sources in the reference's order)
+// element stride es (own value, then the sources in the reference's order);
+// gather_lf_load issues the four loads, gather_sum adds them
 // (FP32 buffers of the mixed form are accumulated in FP64 the same way)
+template <typename T>
+__device__ __forceinline__ void gather_lf_load(const T *f, size_t es, size_t off, int L, int ie, int pt, int k, int s0,
+                                               int s1, int s2, T (&r)[4]) {
+  const size_t own = (size_t)ie * es + off + (size_t)pt * L + k;
+  auto at = [&](int s) { return opaque_index(s >= 0 ? (size_t)(s >> 4) * es + off + (size_t)(s & 15) * L + k : own); };
+  r[0] = f[own];
+  r[1] = f[at(s0)];
+  r[2] = f[at(s1)];
+  r[3] = f[at(s2)];
+}
+
 template <typename T>
 __device__ __forceinline__ double gather_lf(const T *f, size_t es, size_t off, int L, int ie, int pt, int k, int s0,
                                             int s1, int s2) {
-  double x = f[(size_t)ie * es + off + (size_t)pt * L + k];
-  if (s0 >= 0) x += (double)f[(size_t)(s0 >> 4) * es + off + (size_t)(s0 & 15) * L + k];
-  if (s1 >= 0) x += (double)f[(size_t)(s1 >> 4) * es + off + (size_t)(s1 & 15) * L + k];
-  if (s2 >= 0) x += (double)f[(size_t)(s2 >> 4) * es + off + (size_t)(s2 & 15) * L + k];
-  return x;
+  T r[4];
+  gather_lf_load(f, es, off, L, ie, pt, k, s0, s1, s2, r);
+  return gather_sum(r, s0, s1, s2);
 }
 
 // ------------------------------------------------------------ limiters
@@ -213,26 +225,30 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
   const size_t s = slab * 16 + pt, v = slab * 32 + pt;
   const double rhsmdt = a.rhs_multiplier * a.dt;
   double dpproj;
+  const int gkc = gon ? gk : L - 1;  // gathers of a padding lane stay in bounds
   if (GATHER) {
     // assembled divdp_proj of stage 1
-    if (gon) tile[0][gkk][gpt] = gather_lf(a.fdss_pre_in, (size_t)16 * L, 0, L, ie, gpt, gk, g0, g1, g2) * gr;
-    __syncthreads();
+    const double y = gather_lf(a.fdss_pre_in, (size_t)16 * L, 0, L, ie, gpt, gkc, g0, g1, g2) * gr;
+    if (gon) tile[0][gkk][gpt] = y;
+    lds_barrier();
     dpproj = tile[0][kk][pt];
     if (son) a.divdp_proj_w[s] = dpproj;
-    __syncthreads();
+    lds_barrier();
   } else {
     dpproj = a.divdp_proj[s];
   }
   const double dp = a.derived_dp[s] - rhsmdt * dpproj;
   const double vs0 = a.vn0[v] / dp, vs1 = a.vn0[v + 16] / dp;
   double dpdissk = dp - a.dt * a.divdp[s];
-  const bool add_ps_diss = a.nu_p > 0 && a.rhs_viss != 0.0;
+  // (the gathering kernel runs stage 2 only, where rhs_viss == 0: no
+  // biharmonic terms, and no conditional loads that would wait for the gathers)
+  const bool add_ps_diss = !GATHER && a.nu_p > 0 && a.rhs_viss != 0.0;
   if (add_ps_diss) {
     const double diss_fac = -a.rhs_viss * a.dt * a.nu_q;
     dpdissk += diss_fac * a.dpdiss_bih[s] / e.sph;
   }
   if (son) a.fdss_pre[((size_t)ie * 16 + pt) * L + k] = a.fdss[s] * e.sph;  // [ie][pt][lev]
-  const bool add_hv = a.rhs_viss != 0.0;
+  const bool add_hv = !GATHER && a.rhs_viss != 0.0;
   const bool limit = a.limiter_option == 8 || a.limiter_option == 9;
   DvvLaneF df;
   float rmd = 0.f;
@@ -243,24 +259,58 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
   const double c = e.sph * dpdissk;
   const double s1 = limit ? slab_sum_serial(c) : 0.0;  // same for every tracer
   const size_t es = (size_t)Q * 16 * L;
-  double gnext = 0.0;
-  if (GATHER && gon && Q > 0) gnext = gather_lf(a.qdp_pre_in, es, 0, L, ie, gpt, gk, g0, g1, g2);
+  double graw[4] = {};  // the loads of the next tracer's gather
+  if (GATHER && Q > 0) gather_lf_load(a.qdp_pre_in, es, 0, L, ie, gpt, gkc, g0, g1, g2, graw);
   const int kc = son ? k : L - 1;  // reads of a padding slab stay in bounds
+  // FP64 slab path (stages 1 and 3): tracer q+1's Qdp, biharmonic term and
+  // bounds are loaded during tracer q's arithmetic.  The loads are
+  // unconditional (past the last tracer the last is reloaded; without the
+  // biharmonic term Qdp is loaded in its place): a conditional load would
+  // make the waits that follow cover the prefetch.
+  constexpr bool PF = !GATHER && !MIXED;
+  const double *qbp = add_hv ? a.qbih : a.qdp_n0;
+  double pf[4] = {};  // Qdp, biharmonic term, qmin, qmax of the next tracer
+  auto fetch = [&](int q) {
+    const int qq = q < Q ? q : Q - 1;
+    const size_t qi = (((size_t)ie * Q + qq) * L + kc) * 16 + pt;
+    const double *ql = a.qlim + (((size_t)ie * Q + qq) * 2) * L;
+    pf[2] = ql[kc];
+    pf[3] = ql[L + kc];
+    pf[0] = a.qdp_n0[qi];
+    pf[1] = qbp[qi];
+  };
+  if (PF && Q > 0) fetch(0);
   for (int q = 0; q < Q; ++q) {
     const size_t qi = (((size_t)ie * Q + q) * L + kc) * 16 + pt;
-    double qd;
-    if (GATHER) {
-      // the stage-1 exchange of tracer q; tracer q+1's gathers are issued
-      // before this tracer's arithmetic
-      if (gon) tile[q & 1][gkk][gpt] = gnext * gr;
-      if (gon && q + 1 < Q) gnext = gather_lf(a.qdp_pre_in, es, (size_t)(q + 1) * 16 * L, L, ie, gpt, gk, g0, g1, g2);
-      __syncthreads();
-      qd = tile[q & 1][kk][pt];
+    double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
+    double minp, maxp, qd, q0pf = 0.0;
+    if (PF) {
+      qd = pf[0];
+      q0pf = add_hv ? pf[1] : 0.0;
+      minp = pf[2];
+      maxp = pf[3];
+      fetch(q + 1);
     } else {
+      // this tracer's bounds are loaded before the next tracer's gather is
+      // issued: loads complete in issue order, so waiting for them does not
+      // wait for the gather
+      minp = ql[kc];
+      maxp = ql[L + kc];
+    }
+    if (GATHER) {
+      // the stage-1 exchange of tracer q; tracer q+1's loads are issued
+      // before this tracer's arithmetic and stay in flight across the
+      // (LDS-only) barrier
+      const double y = gather_sum(graw, g0, g1, g2) * gr;
+      if (gon) tile[q & 1][gkk][gpt] = y;
+      // (unconditional, the last tracer reloads itself: a branch here would
+      // make every later wait cover the gathers)
+      gather_lf_load(a.qdp_pre_in, es, (size_t)(q + 1 < Q ? q + 1 : q) * 16 * L, L, ie, gpt, gkc, g0, g1, g2, graw);
+      lds_barrier();
+      qd = tile[q & 1][kk][pt];
+    } else if (!PF) {
       qd = a.qdp_n0[qi];
     }
-    double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
-    double minp = ql[kc], maxp = ql[L + kc];
     if (stage == 2) {  // compute_qmin_qmax with rhs_multiplier == 1
       const double xq = qd / dp;
       minp = dmin(minp, slab_min(xq));
@@ -277,8 +327,8 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
       const float dv = (row_contractf(df.J, gv0) + col_contractf(df.I, gv1)) * rmd;
       ptens = (qd + -a.dt * (double)dv + q0);
     } else {
-      const double q0 = add_hv ? (a.bih32 ? (double)((const float *)a.qbih)[qi] : a.qbih[qi]) : 0;
-      ptens = sph_div_update(e, d, -a.dt, vs0, vs1, qd, q0);
+      // (the FP64 path never holds an FP32 biharmonic)
+      ptens = sph_div_update(e, d, -a.dt, vs0, vs1, qd, q0pf);
     }
     bool wmin = stage == 2, wmax = stage == 2;
     if (limit) {
@@ -360,28 +410,33 @@ __global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int avg) {
   const bool gk = x.k < L, wkv = x.k0 + x.wk < L;
   const size_t es = (size_t)Q * 16 * L;
   const int nq = (Q - q0) < QB ? (Q - q0) : QB;
-  double v[QB];
+  const int kc = gk ? x.k : L - 1;  // gathers of a padding lane stay in bounds
+  // every load of the block's gathers is issued before the first sum
+  double v[QB], fr[4];
+  if (nq > 0) {
 #pragma unroll
-  for (int i = 0; i < QB; ++i)
-    if (i < nq && gk) v[i] = gather_lf(a.qdp_pre, es, (size_t)(q0 + i) * 16 * L, L, x.ie, x.pt, x.k, s0, s1, s2);
+    for (int i = 0; i < QB; ++i)
+      v[i] = gather_lf(a.qdp_pre, es, (size_t)(q0 + (i < nq ? i : 0)) * 16 * L, L, x.ie, x.pt, kc, s0, s1, s2);
+  }
+  if (q0 == 0) gather_lf_load(a.fdss_pre, (size_t)16 * L, 0, L, x.ie, x.pt, kc, s0, s1, s2, fr);
 #pragma unroll
   for (int i = 0; i < QB; ++i) {
     if (i >= nq) break;
     double (*t)[17] = tile[i & 1];
     if (gk) t[x.kk][x.pt] = v[i] * r;
-    __syncthreads();
+    lds_barrier();
     if (wkv) {
       const size_t o = (((size_t)x.ie * Q + q0 + i) * L + x.k0 + x.wk) * 16 + x.wpt;
       double y = t[x.wk][x.wpt];
       if (avg) y = (a.qdp_avg_n0[o] + (3.0 - 1) * y) / 3.0;
-      a.qdp_np1[o] = y;
+      st_stream(&a.qdp_np1[o], y);
     }
   }
   if (q0 == 0) {
-    __syncthreads();
-    if (gk) tile[0][x.kk][x.pt] = gather_lf(a.fdss_pre, (size_t)16 * L, 0, L, x.ie, x.pt, x.k, s0, s1, s2) * r;
-    __syncthreads();
-    if (wkv) a.fdss[((size_t)x.ie * L + x.k0 + x.wk) * 16 + x.wpt] = tile[0][x.wk][x.wpt];
+    lds_barrier();
+    if (gk) tile[0][x.kk][x.pt] = gather_sum(fr, s0, s1, s2) * r;
+    lds_barrier();
+    if (wkv) st_stream(&a.fdss[((size_t)x.ie * L + x.k0 + x.wk) * 16 + x.wpt], tile[0][x.wk][x.wpt]);
   }
 }
 
@@ -443,13 +498,15 @@ __global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
   const bool wkv = wk < L;
   const size_t es = (size_t)Q * 16 * L;
   const int nq = (Q - q0) < QB ? (Q - q0) : QB;
+  const int kc = gk ? x.k : L - 1;  // gathers of a padding lane stay in bounds
+  // every load of the block's gathers is issued before the first sum
   double v[QB];
 #pragma unroll
-  for (int i = 0; i < QB; ++i)
-    if (i < nq && gk) {
-      if (MIXED) v[i] = gather_lf((const float *)a.qbih_pre, es, (size_t)(q0 + i) * 16 * L, L, x.ie, x.pt, x.k, s0, s1, s2);
-      else v[i] = gather_lf(a.qbih_pre, es, (size_t)(q0 + i) * 16 * L, L, x.ie, x.pt, x.k, s0, s1, s2);
-    }
+  for (int i = 0; i < QB; ++i) {
+    const size_t off = (size_t)(q0 + (i < nq ? i : 0)) * 16 * L;
+    if (MIXED) v[i] = gather_lf((const float *)a.qbih_pre, es, off, L, x.ie, x.pt, kc, s0, s1, s2);
+    else v[i] = gather_lf(a.qbih_pre, es, off, L, x.ie, x.pt, kc, s0, s1, s2);
+  }
   ElemLane e;
   load_elem_light(a.m, x.ie, x.wpt, e);
   DvvLane d;
@@ -469,20 +526,20 @@ __global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
     if (i >= nq) break;
     double (*t)[17] = tile[i & 1];
     if (gk) t[x.kk][x.pt] = v[i];
-    __syncthreads();
+    lds_barrier();
     // every lane of a slab takes part in the laplacian's cross-lane reads
     const double y = wkv ? t[x.wk][x.wpt] * e.rsmp : 0.0;
     const size_t o = (((size_t)x.ie * Q + q0 + i) * L + wk) * 16 + x.wpt;
     if (MIXED) {
       const double lap = sph_laplacef(ef, df, (float)y);
-      if (wkv) ((float *)a.qbih)[o] = (float)(f * dp0 * lap / e.sph);
+      if (wkv) st_stream(&((float *)a.qbih)[o], (float)(f * dp0 * lap / e.sph));
     } else if (TENSOR) {
       // BIHPostTensorHV (EulerStepFunctorImpl.hpp:281-291)
       const double lap = sph_laplace_tensor(e, d, tv, y);
-      if (wkv) a.qbih[o] = (f * dp0 * lap / e.sph);
+      if (wkv) st_stream(&a.qbih[o], f * dp0 * lap / e.sph);
     } else {
       const double lap = sph_laplace(e, d, y);
-      if (wkv) a.qbih[o] = (f * dp0 * lap / e.sph);
+      if (wkv) st_stream(&a.qbih[o], f * dp0 * lap / e.sph);
     }
   }
 }

@@ -171,16 +171,22 @@ __global__ __launch_bounds__(256) void hv_dss_update_kernel(HvArgs a) {
   const int *src = a.src + ((size_t)ie * 16 + pt) * 3;
   const int s0 = src[0], s1 = src[1], s2 = src[2];
   const size_t es = (size_t)a.nlev * 16, ev = (size_t)a.nlev * 32;
-  auto g = [&](const double *f, size_t estr, size_t lstr, size_t c) {
-    const size_t off = (size_t)k * lstr + c;
-    double y = f[(size_t)ie * estr + off + pt];
-    if (s0 >= 0) y += f[(size_t)(s0 >> 4) * estr + off + (s0 & 15)];
-    if (s1 >= 0) y += f[(size_t)(s1 >> 4) * estr + off + (s1 & 15)];
-    if (s2 >= 0) y += f[(size_t)(s2 >> 4) * estr + off + (s2 & 15)];
-    return y;
+  // the loads of the four gathers are issued before the first sum
+  auto ld = [&](const double *f, size_t estr, size_t lstr, size_t c, double (&r)[4]) {
+    const size_t off = (size_t)k * lstr + c, own = (size_t)ie * estr + off + pt;
+    auto at = [&](int src) { return opaque_index(src >= 0 ? (size_t)(src >> 4) * estr + off + (src & 15) : own); };
+    r[0] = f[own];
+    r[1] = f[at(s0)];
+    r[2] = f[at(s1)];
+    r[3] = f[at(s2)];
   };
-  const double b0 = g(a.vt_a, ev, 32, 0), b1 = g(a.vt_a, ev, 32, 16);
-  const double bt = g(a.tt_a, es, 16, 0), bdp = g(a.dpt_a, es, 16, 0);
+  double r0[4], r1[4], rt[4], rdp[4];
+  ld(a.vt_a, ev, 32, 0, r0);
+  ld(a.vt_a, ev, 32, 16, r1);
+  ld(a.tt_a, es, 16, 0, rt);
+  ld(a.dpt_a, es, 16, 0, rdp);
+  const double b0 = gather_sum(r0, s0, s1, s2), b1 = gather_sum(r1, s0, s1, s2);
+  const double bt = gather_sum(rt, s0, s1, s2), bdp = gather_sum(rdp, s0, s1, s2);
   const double r = a.m.rspheremp[(size_t)ie * 16 + pt];
   const size_t i = ((size_t)ie * a.nlev + k) * 16 + pt;
   const size_t v0 = ((size_t)ie * a.nlev + k) * 32 + pt, v1 = v0 + 16;

@@ -186,6 +186,46 @@ constexpr int REMAP_MAX_NLEV = 256;
 void launch_remap(hipStream_t st, const RemapArgs &a);
 void launch_zero_int(hipStream_t st, int *p);
 
+// Streaming (non-temporal) store for outputs that only a later kernel reads:
+// the line is not kept in L2, so the neighbour copies an exchange kernel
+// gathers stay cached longer.
+template <typename T>
+__device__ __forceinline__ void st_stream(T *p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
+// Block barrier that orders LDS accesses only.  __syncthreads() also waits
+// for every global load in flight; with this one, loads issued before the
+// barrier (the next field's gathers) stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// An index the compiler cannot see through: the selected source index of a
+// gather stays one unconditional load (otherwise the select of addresses is
+// turned back into a branch around the neighbour load and a wait).
+__device__ __forceinline__ size_t opaque_index(size_t i) {
+  asm volatile("" : "+v"(i));
+  return i;
+}
+
+// DSS gather of one point: the own value r[0] and up to three neighbour
+// copies r[1..3] from sources s0, s1, s2 (-1: absent), summed in that order.
+// The loads are unconditional (an absent source re-reads the own value), so
+// all four are in flight together instead of one branch and wait each; the
+// absent terms are dropped by a select, which rounds exactly like the
+// conditional sum.
+template <typename T>
+__device__ __forceinline__ double gather_sum(const T (&r)[4], int s0, int s1, int s2) {
+  double x = r[0];
+  x = s0 >= 0 ? x + (double)r[1] : x;
+  x = s1 >= 0 ? x + (double)r[2] : x;
+  x = s2 >= 0 ? x + (double)r[3] : x;
+  return x;
+}
+
 // XCD-aware block order: consecutive logical blocks stay on one XCD (one L2)
 __device__ __forceinline__ int xcd_block(int b, int n) {
   const int q = n >> 3, r = n & 7, x = b & 7;
