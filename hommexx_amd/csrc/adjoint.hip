@@ -241,10 +241,21 @@ __global__ __launch_bounds__(256) void adj_caar_b1_kernel(AdjCaarArgs a) {
 }
 
 // B2: column reverse scans: geopotential, pressure, eta and omega sums
-__global__ __launch_bounds__(256) void adj_caar_b2_kernel(AdjCaarArgs a) {
-  const size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= (size_t)a.nelem * 16) return;
-  const int ie = (int)(col >> 4), pt = (int)(col & 15), L = a.nlev;
+// The column's levels are split over the 4 waves of a block (64 columns per
+// block): each chunk re-runs the serial sums over the levels above it (phi
+// terms, and the full column sum lstot) or below it (pb, the eta and omega
+// terms), so every sum keeps its order and the result is that of one thread
+// per column, with four times the threads in flight.
+constexpr int B2C = 4;
+__global__ __launch_bounds__(64 * B2C) void adj_caar_b2_kernel(AdjCaarArgs a) {
+  const int chunk = threadIdx.x >> 6;
+  const size_t col = (size_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  const bool on = col < (size_t)a.nelem * 16;
+  const int L = a.nlev, cs = (L + B2C - 1) / B2C;
+  const int k0 = chunk * cs < L ? chunk * cs : L;
+  const int k1 = k0 + cs < L ? k0 + cs : L;
+  const size_t c = on ? col : 0;
+  const int ie = (int)(c >> 4), pt = (int)(c & 15);
   const size_t o = (size_t)ie * L * 16 + pt;
   const double *__restrict__ phib = a.phib + o;
   const double *__restrict__ pc = a.p + o;
@@ -256,29 +267,44 @@ __global__ __launch_bounds__(256) void adj_caar_b2_kernel(AdjCaarArgs a) {
   double *__restrict__ ldp = a.ldp + o;
   double *__restrict__ pb = a.pb + o;
   double *__restrict__ divb = a.divb + o;
-  double phisum = 0.0, lstot = 0.0;
+  double lstot = 0.0;
+  if (on) {
+    double phisum = 0.0;
 #pragma unroll 8
-  for (int k = 0; k < L; ++k) {
-    const size_t i = (size_t)k * 16;
-    const double ph = phib[i];
-    const double lr = ph + 2.0 * phisum;
-    phisum += ph;
-    const double P = pc[i], T = tc[i], dp = dpc[i];
-    lt[i] += RGAS * dp * 0.5 / P * lr;
-    ldp[i] += RGAS * T * 0.5 / P * lr;
-    pb[i] -= RGAS * T * dp * 0.5 / (P * P) * lr;
-    lstot += a.hybi[k] * etab[i];
+    for (int k = 0; k < k0; ++k) phisum += phib[(size_t)k * 16];
+#pragma unroll 8
+    for (int k = 0; k < L; ++k) lstot += a.hybi[k] * etab[(size_t)k * 16];
+#pragma unroll 4
+    for (int k = k0; k < k1; ++k) {
+      const size_t i = (size_t)k * 16;
+      const double ph = phib[i];
+      const double lr = ph + 2.0 * phisum;
+      phisum += ph;
+      const double P = pc[i], T = tc[i], dp = dpc[i];
+      lt[i] += RGAS * dp * 0.5 / P * lr;
+      ldp[i] += RGAS * T * 0.5 / P * lr;
+      pb[i] -= RGAS * T * dp * 0.5 / (P * P) * lr;
+    }
   }
-  double spb = 0.0, scb = 0.0;
+  __syncthreads();  // pb of the levels below each chunk is final
+  if (on) {
+    double spb = 0.0, scb = 0.0;
 #pragma unroll 8
-  for (int k = L - 1; k >= 0; --k) {
-    const size_t i = (size_t)k * 16;
-    const double b = pb[i];
-    ldp[i] += 0.5 * b + spb;
-    spb += b;
-    const double om = ombp[i];
-    divb[i] += scb + lstot - 0.5 * om;
-    scb += -etab[i] - om;
+    for (int k = L - 1; k >= k1; --k) {
+      const size_t i = (size_t)k * 16;
+      spb += pb[i];
+      scb += -etab[i] - ombp[i];
+    }
+#pragma unroll 4
+    for (int k = k1 - 1; k >= k0; --k) {
+      const size_t i = (size_t)k * 16;
+      const double b = pb[i];
+      ldp[i] += 0.5 * b + spb;
+      spb += b;
+      const double om = ombp[i];
+      divb[i] += scb + lstot - 0.5 * om;
+      scb += -etab[i] - om;
+    }
   }
 }
 
@@ -832,7 +858,7 @@ void launch_adj_caar(hipStream_t st, const AdjCaarArgs &a) {
   hipLaunchKernelGGL(adj_caar_div_kernel, dim3(nblk(slab_threads)), dim3(256), 0, st, a);
   hipLaunchKernelGGL(adj_caar_scan_kernel, dim3(nblk(cols)), dim3(256), 0, st, a);
   hipLaunchKernelGGL(adj_caar_b1_kernel, dim3(nblk(slab_threads)), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(adj_caar_b2_kernel, dim3(nblk(cols)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(adj_caar_b2_kernel, dim3(nblk(cols, 64)), dim3(64 * B2C), 0, st, a);
   hipLaunchKernelGGL(adj_caar_b3_kernel, dim3(nblk(slab_threads)), dim3(256), 0, st, a);
 }
 

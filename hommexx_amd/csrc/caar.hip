@@ -64,6 +64,7 @@ __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
   // pressure at the top of the chunk: the serial recurrence over the levels above
   double p_top = a.p_top, dp_top = 0.0;
   if (on)
+#pragma unroll 8
     for (int k = 0; k < k0; ++k) {
       const double dp = a.dp_n0[sb + (size_t)k * 16];
       p_top = p_top + 0.5 * (dp_top + dp);
@@ -71,15 +72,31 @@ __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
     }
 
   // ---------------------------------------------------------------- A
+  // Phases A, B and D load level k+1's inputs while level k is computed (the
+  // loads of y are issued first; x = y at the end waits for them)
+  auto kn = [&](int k) { return k + 1 < k1 ? k + 1 : k; };
   if (on) {
-    for (int k = k0; k < k1; ++k) {
+    double x[5], y[5];  // dp, u, w, vn0 (2)
+    auto ldA = [&](int k, double (&r)[5]) {
       const size_t s = sb + (size_t)k * 16, v = vb + (size_t)k * 32;
-      const double dp = a.dp_n0[s];
-      const double u = a.v_n0[v], w = a.v_n0[v + 16];
+      r[0] = a.dp_n0[s];
+      r[1] = a.v_n0[v];
+      r[2] = a.v_n0[v + 16];
+      r[3] = a.vn0[v];
+      r[4] = a.vn0[v + 16];
+    };
+    if (k0 < k1) ldA(k0, x);
+    for (int k = k0; k < k1; ++k) {
+      ldA(kn(k), y);
+      const size_t v = vb + (size_t)k * 32;
+      const double dp = x[0];
+      const double u = x[1], w = x[2];
       const double vdp0 = u * dp, vdp1 = w * dp;
-      a.vn0[v] += eta * vdp0;
-      a.vn0[v + 16] += eta * vdp1;
+      a.vn0[v] = x[3] + eta * vdp0;
+      a.vn0[v + 16] = x[4] + eta * vdp1;
       divv[(size_t)k * 16] = sph_div(e, d, vdp0, vdp1);
+#pragma unroll
+      for (int i = 0; i < 5; ++i) x[i] = y[i];
     }
   }
   __syncthreads();
@@ -115,11 +132,24 @@ __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
   if (XL) __syncthreads();
   if (on) {
     double p_prev = p_top, dp_prev = dp_top;
-    for (int k = k0; k < k1; ++k) {
+    double x[7], y[7];  // dp, u, w, T, omega, T(nm1), dp3d(nm1)
+    auto ldB = [&](int k, double (&r)[7]) {
       const size_t s = sb + (size_t)k * 16, v = vb + (size_t)k * 32;
-      const double dp = a.dp_n0[s];
-      const double u = a.v_n0[v], w = a.v_n0[v + 16];
-      const double t = a.t_n0[s];
+      r[0] = a.dp_n0[s];
+      r[1] = a.v_n0[v];
+      r[2] = a.v_n0[v + 16];
+      r[3] = a.t_n0[s];
+      r[4] = a.omega[s];
+      r[5] = a.t_nm1[s];
+      r[6] = a.dp_nm1[s];
+    };
+    if (k0 < k1) ldB(k0, x);
+    for (int k = k0; k < k1; ++k) {
+      ldB(kn(k), y);
+      const size_t s = sb + (size_t)k * 16;
+      const double dp = x[0];
+      const double u = x[1], w = x[2];
+      const double t = x[3];
       const double tv = virtual_t(a, qv, k, t, dp);
       const double p = p_prev + 0.5 * (dp_prev + dp);
       p_prev = p;
@@ -137,20 +167,22 @@ __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
       const double vgrad_p = u * gp0 + w * gp1;
       const double om = (vgrad_p - (integ + 0.5 * dvk)) / p;
       integ = integ + dvk;
-      a.omega[s] += eta * om;
+      a.omega[s] = x[4] + eta * om;
       double gt0, gt1;
       sph_grad(e, d, t, gt0, gt1);
       const double vgrad_t = u * gt0 + w * gt1;
       const double ttens = (RS0 ? -tva : 0) - vgrad_t + KAPPA * tv * om;
-      double temp = ttens * dt + a.t_nm1[s];
+      double temp = ttens * dt + x[5];
       temp *= e.sph;
       a.out_t[s] = temp;
       double tmp = ek1;
       tmp += dvk;
       tmp -= ek;
-      tmp = a.dp_nm1[s] - tmp * dt;
+      tmp = x[6] - tmp * dt;
       a.out_dp[s] = e.sph * tmp;
       phi[(size_t)k * 16] = RGAS * tv * (dp * 0.5 / p);
+#pragma unroll
+      for (int i = 0; i < 7; ++i) x[i] = y[i];
     }
   }
   __syncthreads();
@@ -171,11 +203,23 @@ __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
     double integ = 0.0;
     if (RS0)
       for (int k = 0; k < k0; ++k) integ = integ + divv[(size_t)k * 16];
-    for (int k = k0; k < k1; ++k) {
+    double x[6], y[6];  // dp, u, w, T, v(nm1) (2)
+    auto ldD = [&](int k, double (&r)[6]) {
       const size_t s = sb + (size_t)k * 16, v = vb + (size_t)k * 32;
-      const double dp = a.dp_n0[s];
-      const double u = a.v_n0[v], w = a.v_n0[v + 16];
-      const double t = a.t_n0[s];
+      r[0] = a.dp_n0[s];
+      r[1] = a.v_n0[v];
+      r[2] = a.v_n0[v + 16];
+      r[3] = a.t_n0[s];
+      r[4] = a.v_nm1[v];
+      r[5] = a.v_nm1[v + 16];
+    };
+    if (k0 < k1) ldD(k0, x);
+    for (int k = k0; k < k1; ++k) {
+      ldD(kn(k), y);
+      const size_t v = vb + (size_t)k * 32;
+      const double dp = x[0];
+      const double u = x[1], w = x[2];
+      const double t = x[3];
       const double tv = virtual_t(a, qv, k, t, dp);
       const double p = p_prev + 0.5 * (dp_prev + dp);
       p_prev = p;
@@ -201,11 +245,13 @@ __global__ __launch_bounds__(64 * CNC) void caar_kernel(CaarArgs a) {
       eg1 *= -1;
       eg1 += (RS0 ? -va1 : 0) - u * vort;
       eg0 *= dt;
-      eg0 += a.v_nm1[v];
+      eg0 += x[4];
       eg1 *= dt;
-      eg1 += a.v_nm1[v + 16];
+      eg1 += x[5];
       a.out_v[v] = e.sph * eg0;
       a.out_v[v + 16] = e.sph * eg1;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) x[i] = y[i];
     }
   }
 }

@@ -59,16 +59,23 @@ __global__ __launch_bounds__(256) void euler_divdp_qmm_kernel(EulerArgs a) {
   a.divdp_proj_w[s] = dv;
   const double rhsmdt = a.rhs_multiplier * a.dt;
   const double dp = a.derived_dp[s] - rhsmdt * dv;
-  for (int q = 0; q < Q; ++q) {
-    const size_t qi = (((size_t)ie * Q + q) * L + k) * 16 + pt;
-    const double x = a.qdp_n0[qi] / dp;
+  // tracer q+1's Qdp is loaded before tracer q's reductions; every lane
+  // stores the slab's bounds (the same value: measured faster here than a
+  // store from one lane, whose branch makes the next wait cover the stores)
+  const size_t q0i = ((size_t)ie * Q * L + k) * 16 + pt, qs = (size_t)L * 16;
+  double nx = Q > 0 ? a.qdp_n0[q0i] : 0.0;
+  auto tracer = [&](int q) {
+    const double x = nx / dp;
+    nx = a.qdp_n0[q0i + (size_t)(q + 1 < Q ? q + 1 : q) * qs];
     const double mn = slab_min(x), mx = slab_max(x);
-    if (pt == 0) {
-      double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
-      ql[k] = mn;
-      ql[L + k] = mx;
-    }
-  }
+    double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
+    ql[k] = mn;
+    ql[L + k] = mx;
+  };
+  // the first tracer is peeled so that on both paths into the loop the same
+  // stores follow the prefetch (the wait then covers the prefetch only)
+  if (Q > 0) tracer(0);
+  for (int q = 1; q < Q; ++q) tracer(q);
 }
 
 // min/max of qlim over the element and its (up to 8) neighbours (:487-507).
@@ -464,21 +471,29 @@ __global__ __launch_bounds__(256) void euler_qmm_lap_kernel(EulerArgs a) {
     load_elemf(a.m, ie, pt, ef);
     load_dvvf(a.m.dvv, pt, df);
   }
-  for (int q = 0; q < Q; ++q) {
-    const size_t qi = (((size_t)ie * Q + q) * L + k) * 16 + pt;
-    const double x = a.qdp_n0[qi] / dp;
+  if (Q <= 0) return;
+  // tracer q+1's Qdp is loaded before tracer q's arithmetic
+  const size_t q0i = ((size_t)ie * Q * L + k) * 16 + pt, qs = (size_t)L * 16;
+  const bool scale = a.nu_p > 0;
+  double nx = a.qdp_n0[q0i];
+  auto tracer = [&](int q) {
+    const double x = nx / dp;
+    nx = a.qdp_n0[q0i + (size_t)(q + 1 < Q ? q + 1 : q) * qs];
     const double mn = slab_min(x), mx = slab_max(x);
     if (pt == 0) {
       double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
       ql[k] = mn;
       ql[L + k] = mx;
     }
-    double y = x;
-    if (a.nu_p > 0) y = y * wdp / dp0;
+    const double y = scale ? x * wdp / dp0 : x;
     const size_t o = (((size_t)ie * Q + q) * 16 + pt) * L + k;  // [ie][q][pt][lev]
     if (MIXED) ((float *)a.qbih_pre)[o] = sph_laplacef(ef, df, (float)y);
     else a.qbih_pre[o] = sph_laplace(e, d, y);
-  }
+  };
+  // the first tracer is peeled so that on both paths into the loop the same
+  // stores follow the prefetch (the wait then covers the prefetch only)
+  tracer(0);
+  for (int q = 1; q < Q; ++q) tracer(q);
 }
 
 // biharmonic, second half (:216-310): DSS of the first laplacian (with

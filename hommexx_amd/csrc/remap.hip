@@ -212,12 +212,13 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
   double dma_j[RFB], ai_prev[RFB], var_prefix[RFB], mass_prev[RFB];
   double cc0[RFB], cc1[RFB], cc2[RFB];
   double pv[RFB], qv[RFB];  // field values of cells j+3, j+4 (loaded ahead)
+  // unconditional loads (past the bottom the last cell is reloaded; an
+  // absent field reads its stand-in pointer): with a branch here every later
+  // wait would also cover the loads just issued
   auto fetch = [&](int m, double *v) {
-    if (m <= L) {
+    const size_t c = (size_t)((m <= L ? m : L) - 1);
 #pragma unroll
-      for (int i = 0; i < RFB; ++i)
-        if (on[i]) v[i] = var[i][(size_t)(m - 1) * vs[i]];
-    }
+    for (int i = 0; i < RFB; ++i) v[i] = var[i][c * vs[i]];
   };
   if (active) {
     const double s0 = src[0], s1 = src[16];
