@@ -40,16 +40,47 @@ __global__ __launch_bounds__(256) void hv_lap1_kernel(HvArgs a) {
   a.vt_a[v + 16] = l1;
 }
 
+// The first exchange is fused in: each lane gathers the assembled first
+// laplacian of its point (own value and neighbour copies of the vt_a, tt_a,
+// dpt_a outputs of hv_lap1, in dss_kernel's order, times rspheremp) and the
+// outputs go to vt_b, tt_b, dpt_b, which hv_dss_update then exchanges (the
+// inputs of a neighbour's gather are never overwritten during the launch).
+// Slabs are walked in level groups (group_slab) so neighbour copies come
+// from L2.
 template <bool TENSOR>
 __global__ __launch_bounds__(256) void hv_lap2_pre_kernel(HvArgs a) {
   int ie, k, pt;
-  if (!slab_of(gridDim.x, a.nelem, a.nlev, ie, k, pt)) return;
+  if (!group_slab(a.nelem, a.nlev, ie, k, pt)) return;
+  const size_t s = ((size_t)ie * a.nlev + k) * 16 + pt;
+  const size_t v = ((size_t)ie * a.nlev + k) * 32 + pt;
+  double g[4];  // assembled vt (2 components), tt, dpt
+  {
+    const int *src = a.src + ((size_t)ie * 16 + pt) * 3;
+    const int s0 = src[0], s1 = src[1], s2 = src[2];
+    const size_t es = (size_t)a.nlev * 16, ev = (size_t)a.nlev * 32;
+    auto ld = [&](const double *f, size_t estr, size_t lstr, size_t c, double (&r)[4]) {
+      const size_t off = (size_t)k * lstr + c, own = (size_t)ie * estr + off + pt;
+      auto at = [&](int sr) { return opaque_index(sr >= 0 ? (size_t)(sr >> 4) * estr + off + (sr & 15) : own); };
+      r[0] = f[own];
+      r[1] = f[at(s0)];
+      r[2] = f[at(s1)];
+      r[3] = f[at(s2)];
+    };
+    double r0[4], r1[4], rt[4], rdp[4];
+    ld(a.vt_a, ev, 32, 0, r0);
+    ld(a.vt_a, ev, 32, 16, r1);
+    ld(a.tt_a, es, 16, 0, rt);
+    ld(a.dpt_a, es, 16, 0, rdp);
+    const double rs = a.m.rspheremp[(size_t)ie * 16 + pt];
+    g[0] = gather_sum(r0, s0, s1, s2) * rs;
+    g[1] = gather_sum(r1, s0, s1, s2) * rs;
+    g[2] = gather_sum(rt, s0, s1, s2) * rs;
+    g[3] = gather_sum(rdp, s0, s1, s2) * rs;
+  }
   ElemLane e;
   load_elem(a.m, ie, pt, e);
   DvvLane d;
   load_dvv(a.m.dvv, pt, d);
-  const size_t s = ((size_t)ie * a.nlev + k) * 16 + pt;
-  const size_t v = ((size_t)ie * a.nlev + k) * 32 + pt;
   // second laplacian: constant coefficient, or the tensor form
   // (HyperviscosityFunctorImpl.hpp:88-128), then the pre-exchange scaling
   // with the nu_top sponge on the top three levels; each field is finished
@@ -60,8 +91,8 @@ __global__ __launch_bounds__(256) void hv_lap2_pre_kernel(HvArgs a) {
   const double sc = (k == 0 ? 4.0 : k == 1 ? 2.0 : 1.0) * a.nu_top;
   {
     double vt0, vt1;
-    if (TENSOR) sph_vlaplace_cartesian(e, d, tv, vs, a.vt_b[v], a.vt_b[v + 16], vt0, vt1);
-    else sph_vlaplace_contra(e, d, a.nu_ratio2, a.vt_b[v], a.vt_b[v + 16], vt0, vt1);
+    if (TENSOR) sph_vlaplace_cartesian(e, d, tv, vs, g[0], g[1], vt0, vt1);
+    else sph_vlaplace_contra(e, d, a.nu_ratio2, g[0], g[1], vt0, vt1);
     vt0 *= -a.nu;
     vt1 *= -a.nu;
     if (top) {
@@ -70,24 +101,24 @@ __global__ __launch_bounds__(256) void hv_lap2_pre_kernel(HvArgs a) {
       vt0 += sc * lv0;
       vt1 += sc * lv1;
     }
-    a.vt_a[v] = vt0;
-    a.vt_a[v + 16] = vt1;
+    a.vt_b[v] = vt0;
+    a.vt_b[v + 16] = vt1;
   }
   {
-    double tt = TENSOR ? sph_laplace_tensor(e, d, tv, a.tt_b[s]) : sph_laplace(e, d, a.tt_b[s]);
+    double tt = TENSOR ? sph_laplace_tensor(e, d, tv, g[2]) : sph_laplace(e, d, g[2]);
     tt *= -a.nu_s;
     if (top) tt += sc * sph_laplace(e, d, a.t[s]);
-    a.tt_a[s] = tt;
+    a.tt_b[s] = tt;
   }
   const double dpn = a.dp[s];
-  double dpt = TENSOR ? sph_laplace_tensor(e, d, tv, a.dpt_b[s]) : sph_laplace(e, d, a.dpt_b[s]);
+  double dpt = TENSOR ? sph_laplace_tensor(e, d, tv, g[3]) : sph_laplace(e, d, g[3]);
   a.dpdiss_ave[s] += a.eta_ave_w * dpn / a.nsub;
   a.dpdiss_bih[s] += a.eta_ave_w * dpt / a.nsub;
   dpt *= -a.nu_p;
   if (top) dpt += sc * sph_laplace(e, d, dpn);
   dpt *= a.dt;
   dpt += dpn * e.sph;
-  a.dpt_a[s] = dpt;
+  a.dpt_b[s] = dpt;
 }
 
 // ---------------------------------------------------------- mixed form
@@ -204,8 +235,21 @@ __global__ __launch_bounds__(256) void hv_dss_update_kernel(HvArgs a) {
 void launch_hv_cycle(hipStream_t st, const HvArgs &a) {
   const size_t nslab = (size_t)a.nelem * a.nlev;
   const int blocks = (int)((nslab + 15) / 16);
-  if (a.mixed) hipLaunchKernelGGL(hv_lap1_mixed_kernel, dim3(blocks), dim3(256), 0, st, a);
-  else hipLaunchKernelGGL(hv_lap1_kernel, dim3(blocks), dim3(256), 0, st, a);
+  const dim3 gg(group_blocks_x(a.nelem, a.nlev));
+  if (!a.mixed) {
+    // FP64: the first exchange is gathered by hv_lap2_pre, which writes the
+    // second laplacian to the _b buffers; the update exchanges those
+    hipLaunchKernelGGL(hv_lap1_kernel, dim3(blocks), dim3(256), 0, st, a);
+    if (a.consthv) hipLaunchKernelGGL(hv_lap2_pre_kernel<false>, gg, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(hv_lap2_pre_kernel<true>, gg, dim3(256), 0, st, a);
+    HvArgs b = a;
+    b.vt_a = a.vt_b;
+    b.tt_a = a.tt_b;
+    b.dpt_a = a.dpt_b;
+    hipLaunchKernelGGL(hv_dss_update_kernel, gg, dim3(256), 0, st, b);
+    return;
+  }
+  hipLaunchKernelGGL(hv_lap1_mixed_kernel, dim3(blocks), dim3(256), 0, st, a);
   DssArgs x{};
   x.nf = 4;
   x.nelem = a.nelem;
@@ -217,17 +261,13 @@ void launch_hv_cycle(hipStream_t st, const HvArgs &a) {
   x.f[1] = {a.vt_a + 16, a.vt_b + 16, ev, 32};
   x.f[2] = {a.tt_a, a.tt_b, es, 16};
   x.f[3] = {a.dpt_a, a.dpt_b, es, 16};
-  x.f32 = a.mixed;
-  if (a.mixed) {
-    // the first exchange's buffers hold floats: the second velocity
-    // component starts 16 floats in
-    x.f[1] = {(const double *)((const float *)a.vt_a + 16), (double *)((float *)a.vt_b + 16), ev, 32};
-  }
+  // mixed form: the first exchange's buffers hold floats (the second
+  // velocity component starts 16 floats in), exchanged by dss_kernel
+  x.f32 = 1;
+  x.f[1] = {(const double *)((const float *)a.vt_a + 16), (double *)((float *)a.vt_b + 16), ev, 32};
   launch_dss(st, x);
-  if (a.mixed) hipLaunchKernelGGL(hv_lap2_pre_mixed_kernel, dim3(blocks), dim3(256), 0, st, a);
-  else if (a.consthv) hipLaunchKernelGGL(hv_lap2_pre_kernel<false>, dim3(blocks), dim3(256), 0, st, a);
-  else hipLaunchKernelGGL(hv_lap2_pre_kernel<true>, dim3(blocks), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(hv_dss_update_kernel, dim3(group_blocks_x(a.nelem, a.nlev)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(hv_lap2_pre_mixed_kernel, dim3(blocks), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(hv_dss_update_kernel, gg, dim3(256), 0, st, a);
 }
 
 }  // namespace hx
