@@ -82,7 +82,8 @@ __global__ __launch_bounds__(256) void euler_divdp_qmm_kernel(EulerArgs a) {
 // One thread per (tracer, element, level), tracer slowest, so a neighbour's
 // qlim row is still in L2 when the next element row reads it.
 __global__ __launch_bounds__(256) void euler_nbr_minmax_kernel(EulerArgs a) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-aware block order: neighbouring elements' rows are read through one L2
+  const size_t i = (size_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int L = a.nlev, Q = a.qsize;
   if (i >= (size_t)a.nelem * Q * L) return;
   const int k = (int)(i % L);
