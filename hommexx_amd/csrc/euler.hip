@@ -270,14 +270,17 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
   double graw[4] = {};  // the loads of the next tracer's gather
   if (GATHER && Q > 0) gather_lf_load(a.qdp_pre_in, es, 0, L, ie, gpt, gkc, g0, g1, g2, graw);
   const int kc = son ? k : L - 1;  // reads of a padding slab stay in bounds
-  // FP64 slab path (stages 1 and 3): tracer q+1's Qdp, biharmonic term and
+  // Slab path (stages 1 and 3): tracer q+1's Qdp, biharmonic term and
   // bounds are loaded during tracer q's arithmetic.  The loads are
-  // unconditional (past the last tracer the last is reloaded; without the
-  // biharmonic term Qdp is loaded in its place): a conditional load would
-  // make the waits that follow cover the prefetch.
-  constexpr bool PF = !GATHER && !MIXED;
-  const double *qbp = add_hv ? a.qbih : a.qdp_n0;
-  double pf[4] = {};  // Qdp, biharmonic term, qmin, qmax of the next tracer
+  // unconditional (past the last tracer the last is reloaded; without a
+  // biharmonic term of a type, Qdp, or one FP32 value, is loaded in its
+  // place): a conditional load would make the waits that follow cover the
+  // prefetch.
+  constexpr bool PF = !GATHER;
+  const bool bihf = add_hv && a.bih32;  // FP32 biharmonic term (mixed form)
+  const double *qbp = add_hv && !a.bih32 ? a.qbih : a.qdp_n0;
+  const float *qbf = (const float *)(bihf ? a.qbih : a.qdp_n0);
+  double pf[5] = {};  // Qdp, biharmonic term (FP64, FP32), qmin, qmax of the next tracer
   auto fetch = [&](int q) {
     const int qq = q < Q ? q : Q - 1;
     const size_t qi = (((size_t)ie * Q + qq) * L + kc) * 16 + pt;
@@ -286,6 +289,7 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
     pf[3] = ql[L + kc];
     pf[0] = a.qdp_n0[qi];
     pf[1] = qbp[qi];
+    if (MIXED) pf[4] = qbf[bihf ? qi : 0];
   };
   if (PF && Q > 0) fetch(0);
   for (int q = 0; q < Q; ++q) {
@@ -294,7 +298,7 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
     double minp, maxp, qd, q0pf = 0.0;
     if (PF) {
       qd = pf[0];
-      q0pf = add_hv ? pf[1] : 0.0;
+      q0pf = add_hv ? (bihf ? pf[4] : pf[1]) : 0.0;
       minp = pf[2];
       maxp = pf[3];
       fetch(q + 1);
@@ -328,14 +332,13 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
     if (MIXED) {
       // flux divergence in FP32 with compensated contractions; the update
       // qdp - dt*div + qbih stays FP64
-      const double q0 = add_hv ? (a.bih32 ? (double)((const float *)a.qbih)[qi] : a.qbih[qi]) : 0;
+      const double q0 = PF ? q0pf : (add_hv ? (a.bih32 ? (double)((const float *)a.qbih)[qi] : a.qbih[qi]) : 0);
       const double fa = vs0 * qd, fb = vs1 * qd;
       const float gv0 = (float)((e.Dinv[0] * fa + e.Dinv[2] * fb) * e.metdet);
       const float gv1 = (float)((e.Dinv[1] * fa + e.Dinv[3] * fb) * e.metdet);
       const float dv = (row_contractf(df.J, gv0) + col_contractf(df.I, gv1)) * rmd;
       ptens = (qd + -a.dt * (double)dv + q0);
     } else {
-      // (the FP64 path never holds an FP32 biharmonic)
       ptens = sph_div_update(e, d, -a.dt, vs0, vs1, qd, q0pf);
     }
     bool wmin = stage == 2, wmax = stage == 2;

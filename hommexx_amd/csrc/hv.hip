@@ -146,20 +146,48 @@ __global__ __launch_bounds__(256) void hv_lap1_mixed_kernel(HvArgs a) {
   ((float *)a.vt_a)[v + 16] = l1;
 }
 
+// As hv_lap2_pre: the first exchange of the FP32 first laplacian is gathered
+// here (summed in FP64, times rspheremp, rounded to FP32 like the exchanged
+// array) and the outputs go to the _b buffers.
 __global__ __launch_bounds__(256) void hv_lap2_pre_mixed_kernel(HvArgs a) {
   int ie, k, pt;
-  if (!slab_of(gridDim.x, a.nelem, a.nlev, ie, k, pt)) return;
+  if (!group_slab(a.nelem, a.nlev, ie, k, pt)) return;
+  const size_t s = ((size_t)ie * a.nlev + k) * 16 + pt;
+  const size_t v = ((size_t)ie * a.nlev + k) * 32 + pt;
+  float g[4];  // assembled vt (2 components), tt, dpt
+  {
+    const int *src = a.src + ((size_t)ie * 16 + pt) * 3;
+    const int s0 = src[0], s1 = src[1], s2 = src[2];
+    const size_t es = (size_t)a.nlev * 16, ev = (size_t)a.nlev * 32;
+    auto ld = [&](const double *fd, size_t estr, size_t lstr, size_t c, float (&r)[4]) {
+      const float *f = (const float *)fd;
+      const size_t off = (size_t)k * lstr + c, own = (size_t)ie * estr + off + pt;
+      auto at = [&](int sr) { return opaque_index(sr >= 0 ? (size_t)(sr >> 4) * estr + off + (sr & 15) : own); };
+      r[0] = f[own];
+      r[1] = f[at(s0)];
+      r[2] = f[at(s1)];
+      r[3] = f[at(s2)];
+    };
+    float r0[4], r1[4], rt[4], rdp[4];
+    ld(a.vt_a, ev, 32, 0, r0);
+    ld(a.vt_a, ev, 32, 16, r1);
+    ld(a.tt_a, es, 16, 0, rt);
+    ld(a.dpt_a, es, 16, 0, rdp);
+    const double rs = a.m.rspheremp[(size_t)ie * 16 + pt];
+    g[0] = (float)(gather_sum(r0, s0, s1, s2) * rs);
+    g[1] = (float)(gather_sum(r1, s0, s1, s2) * rs);
+    g[2] = (float)(gather_sum(rt, s0, s1, s2) * rs);
+    g[3] = (float)(gather_sum(rdp, s0, s1, s2) * rs);
+  }
   ElemLaneF e;
   load_elemf(a.m, ie, pt, e);
   DvvLaneF d;
   load_dvvf(a.m.dvv, pt, d);
   const double sph = a.m.spheremp[(size_t)ie * 16 + pt];
-  const size_t s = ((size_t)ie * a.nlev + k) * 16 + pt;
-  const size_t v = ((size_t)ie * a.nlev + k) * 32 + pt;
-  double tt = sph_laplacef(e, d, ((const float *)a.tt_b)[s]);
-  double dpt = sph_laplacef(e, d, ((const float *)a.dpt_b)[s]);
+  double tt = sph_laplacef(e, d, g[2]);
+  double dpt = sph_laplacef(e, d, g[3]);
   float f0, f1;
-  sph_vlaplace_contraf(e, d, (float)a.nu_ratio2, ((const float *)a.vt_b)[v], ((const float *)a.vt_b)[v + 16], f0, f1);
+  sph_vlaplace_contraf(e, d, (float)a.nu_ratio2, g[0], g[1], f0, f1);
   double vt0 = f0, vt1 = f1;
   const double dpn = a.dp[s];
   a.dpdiss_ave[s] += a.eta_ave_w * dpn / a.nsub;
@@ -188,10 +216,10 @@ __global__ __launch_bounds__(256) void hv_lap2_pre_mixed_kernel(HvArgs a) {
   }
   dpt *= a.dt;
   dpt += dpn * sph;
-  a.vt_a[v] = vt0;
-  a.vt_a[v + 16] = vt1;
-  a.tt_a[s] = tt;
-  a.dpt_a[s] = dpt;
+  a.vt_b[v] = vt0;
+  a.vt_b[v + 16] = vt1;
+  a.tt_b[s] = tt;
+  a.dpt_b[s] = dpt;
 }
 
 // Second exchange fused with the update: the assembled tensors are used at
@@ -236,38 +264,18 @@ void launch_hv_cycle(hipStream_t st, const HvArgs &a) {
   const size_t nslab = (size_t)a.nelem * a.nlev;
   const int blocks = (int)((nslab + 15) / 16);
   const dim3 gg(group_blocks_x(a.nelem, a.nlev));
-  if (!a.mixed) {
-    // FP64: the first exchange is gathered by hv_lap2_pre, which writes the
-    // second laplacian to the _b buffers; the update exchanges those
-    hipLaunchKernelGGL(hv_lap1_kernel, dim3(blocks), dim3(256), 0, st, a);
-    if (a.consthv) hipLaunchKernelGGL(hv_lap2_pre_kernel<false>, gg, dim3(256), 0, st, a);
-    else hipLaunchKernelGGL(hv_lap2_pre_kernel<true>, gg, dim3(256), 0, st, a);
-    HvArgs b = a;
-    b.vt_a = a.vt_b;
-    b.tt_a = a.tt_b;
-    b.dpt_a = a.dpt_b;
-    hipLaunchKernelGGL(hv_dss_update_kernel, gg, dim3(256), 0, st, b);
-    return;
-  }
-  hipLaunchKernelGGL(hv_lap1_mixed_kernel, dim3(blocks), dim3(256), 0, st, a);
-  DssArgs x{};
-  x.nf = 4;
-  x.nelem = a.nelem;
-  x.nlev = a.nlev;
-  x.src = a.src;
-  x.rsmp = a.m.rspheremp;
-  const long long es = (long long)a.nlev * 16, ev = (long long)a.nlev * 32;
-  x.f[0] = {a.vt_a, a.vt_b, ev, 32};
-  x.f[1] = {a.vt_a + 16, a.vt_b + 16, ev, 32};
-  x.f[2] = {a.tt_a, a.tt_b, es, 16};
-  x.f[3] = {a.dpt_a, a.dpt_b, es, 16};
-  // mixed form: the first exchange's buffers hold floats (the second
-  // velocity component starts 16 floats in), exchanged by dss_kernel
-  x.f32 = 1;
-  x.f[1] = {(const double *)((const float *)a.vt_a + 16), (double *)((float *)a.vt_b + 16), ev, 32};
-  launch_dss(st, x);
-  hipLaunchKernelGGL(hv_lap2_pre_mixed_kernel, dim3(blocks), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(hv_dss_update_kernel, gg, dim3(256), 0, st, a);
+  // the first exchange is gathered by hv_lap2_pre, which writes the second
+  // laplacian to the _b buffers; the update exchanges those
+  if (a.mixed) hipLaunchKernelGGL(hv_lap1_mixed_kernel, dim3(blocks), dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(hv_lap1_kernel, dim3(blocks), dim3(256), 0, st, a);
+  if (a.mixed) hipLaunchKernelGGL(hv_lap2_pre_mixed_kernel, gg, dim3(256), 0, st, a);
+  else if (a.consthv) hipLaunchKernelGGL(hv_lap2_pre_kernel<false>, gg, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(hv_lap2_pre_kernel<true>, gg, dim3(256), 0, st, a);
+  HvArgs b = a;
+  b.vt_a = a.vt_b;
+  b.tt_a = a.tt_b;
+  b.dpt_a = a.dpt_b;
+  hipLaunchKernelGGL(hv_dss_update_kernel, gg, dim3(256), 0, st, b);
 }
 
 }  // namespace hx
