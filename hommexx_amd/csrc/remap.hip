@@ -132,9 +132,6 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
   // the sweep, and without the wait the compiler's waits inside the sweep
   // assume it may still be in flight and also wait for the prefetched loads
   __builtin_amdgcn_s_waitcnt(0x0F70);
-  auto tgt_of = [&](int k) {
-    return (a.hyai[k + 1] - a.hyai[k]) * a.ps0 + (a.hybi[k + 1] - a.hybi[k]) * ps;
-  };
   // dx of Fortran cell m = 1..L with the mirrored ghosts dx(0) = dx(1),
   // dx(-1) = dx(2), dx(L+1) = dx(L), dx(L+2) = dx(L-1)
   auto dxf = [&](int m) {
@@ -148,21 +145,30 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
     dab[2 * k] = a.hyai[k + 1] - a.hyai[k];
     dab[2 * k + 1] = a.hybi[k + 1] - a.hybi[k];
   }
-  if (row == 0) {
+  // The element's source thicknesses go through LDS (the grid chunk's space,
+  // free until the sweep) when they fit, so the serial search below waits on
+  // LDS rather than on one global load per step.
+  const bool lsrc = L <= RCH * RG;
+  if (lsrc) {
+    const double *s0 = (a.nstates ? a.dp : a.src) + (size_t)ie * L * 16;
+    for (int i = threadIdx.x; i < L * 16; i += blockDim.x) gsh[i] = s0[i];
+  }
+  __syncthreads();
+  auto search = [&](auto sat) {  // sat(c): thickness of source cell c of this column
     double pio_L = 0.0;  // pio(L): serial sum of the source thicknesses
-    for (int c = 0; c < L; ++c) pio_L += src[(size_t)c * 16];
+    for (int c = 0; c < L; ++c) pio_L += sat(c);
     double pin_acc = 0.0, pio_si = 0.0, pio_m1 = 0.0, pio_m2 = 0.0;  // pio(si), pio(si-1), pio(si-2)
     int si = 0;
     auto advance_si = [&]() {
       pio_m2 = pio_m1;
       pio_m1 = pio_si;
-      pio_si = si < L ? pio_si + src[(size_t)si * 16] : pio_si + 1.0;  // pio(L+1) = pio(L) + 1
+      pio_si = si < L ? pio_si + sat(si) : pio_si + 1.0;  // pio(L+1) = pio(L) + 1
       ++si;
     };
     for (int k = 0; k < L; ++k) {
       double pin_k1;
       if (k + 1 < L) {
-        pin_acc += tgt_of(k);
+        pin_acc += dab[2 * k] * a.ps0 + dab[2 * k + 1] * ps;  // the target thickness of level k
         pin_k1 = pin_acc;
       } else {
         pin_k1 = pio_L;  // pin(L) = pio(L)
@@ -178,8 +184,12 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
       }
       const int kid = kk - 1;
       ksh[k * 16 + pt] = (short)kid;
-      zsh[k * 16 + pt] = kid >= 0 ? (pin_k1 - (plo + phi) * 0.5) / src[(size_t)kid * 16] : 0.0;
+      zsh[k * 16 + pt] = kid >= 0 ? (pin_k1 - (plo + phi) * 0.5) / sat(kid) : 0.0;
     }
+  };
+  if (row == 0) {
+    if (lsrc) search([&](int c) { return gsh[c * 16 + pt]; });
+    else search([&](int c) { return src[(size_t)c * 16]; });
   }
 
   // ------------------------------------------------------------ fields
