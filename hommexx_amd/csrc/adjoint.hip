@@ -540,9 +540,15 @@ __device__ __forceinline__ int ghost_src(int i, int L) {
 }
 
 // grid of one column: target thicknesses, partitions, target search
-__global__ __launch_bounds__(256) void adj_remap_grid_kernel(AdjRemapArgs a) {
+// The target search walks pio(0..L+1) of its column serially; the column's
+// pio values are also kept in LDS (pio_s[(L+2) rows][blockDim.x], 16 to 64
+// columns per block so they fit in 64 KB), so each search step waits on LDS
+// rather than on a global load of the scratch array.
+__global__ __launch_bounds__(64) void adj_remap_grid_kernel(AdjRemapArgs a) {
+  extern __shared__ double pio_s[];
+  const int bs = blockDim.x, tid = threadIdx.x;
   const size_t ncol = (size_t)a.nelem * 16;
-  const size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t col = (size_t)blockIdx.x * bs + tid;
   if (col >= ncol) return;
   const int L = a.nlev, ie = (int)(col >> 4), pt = (int)(col & 15);
   const RCol c{&a, col, ncol, (size_t)L + 4};
@@ -555,25 +561,30 @@ __global__ __launch_bounds__(256) void adj_remap_grid_kernel(AdjRemapArgs a) {
     const double tg = (a.hyai[k + 1] - a.hyai[k]) * a.ps0 + (a.hybi[k + 1] - a.hybi[k]) * ps;
     c.g(RG_TG, k) = tg;
     c.g(RG_PIO, k) = acc;
+    pio_s[k * bs + tid] = acc;
     c.g(RG_PIN, k) = accn;
     acc += src(k);
     accn += tg;
   }
-  const double pioL = c.g(RG_PIO, L - 1) + src(L - 1);
+  const double pioL = acc;  // pio(L-1) + dx(L-1)
   c.g(RG_PIO, L) = pioL;
   c.g(RG_PIO, L + 1) = pioL + 1.0;
   c.g(RG_PIN, L) = pioL;
+  pio_s[L * bs + tid] = pioL;
+  pio_s[(L + 1) * bs + tid] = pioL + 1.0;
+  double pin = 0.0;  // pin(k+1), the same running sum as above
   for (int k = 0; k < L; ++k) {
-    const double pk = c.g(RG_PIN, k + 1);
+    pin += (a.hyai[k + 1] - a.hyai[k]) * a.ps0 + (a.hybi[k + 1] - a.hybi[k]) * ps;
+    const double pk = k + 1 < L ? pin : pioL;
     int kk = k + 1;
-    while (kk - 1 <= L + 1 && c.g(RG_PIO, kk - 1) <= pk) kk++;
+    while (kk - 1 <= L + 1 && pio_s[(kk - 1) * bs + tid] <= pk) kk++;
     kk--;
     if (kk == L + 1) kk = L;
     // kid >= 0 for every valid column (pio(0) = 0 <= pin(1)); the clamp keeps
     // the reads in bounds for a column the forward step rejected (NaN)
     const int kid = kk - 1 < 0 ? 0 : kk - 1;
     c.g(RG_KID, k) = (double)kid;
-    c.g(RG_Z2, k) = (pk - (c.g(RG_PIO, kid) + c.g(RG_PIO, kid + 1)) * 0.5) / src(kid);
+    c.g(RG_Z2, k) = (pk - (pio_s[kid * bs + tid] + pio_s[(kid + 1) * bs + tid]) * 0.5) / src(kid);
   }
 }
 
@@ -880,7 +891,11 @@ void launch_adj_ps(hipStream_t st, int nelem, int nlev, const double *hybi, cons
 }
 void launch_adj_remap(hipStream_t st, const AdjRemapArgs &a) {
   const size_t ncol = (size_t)a.nelem * 16;
-  hipLaunchKernelGGL(adj_remap_grid_kernel, dim3(nblk(ncol)), dim3(256), 0, st, a);
+  // columns per grid block: as many (up to 64) as keep pio_s within 64 KB
+  int gbs = 64;
+  while (gbs > 1 && (size_t)(a.nlev + 2) * gbs * sizeof(double) > 65536) gbs >>= 1;
+  hipLaunchKernelGGL(adj_remap_grid_kernel, dim3(nblk(ncol, gbs)), dim3(gbs), (size_t)(a.nlev + 2) * gbs * sizeof(double),
+                     st, a);
   hipLaunchKernelGGL(adj_remap_field_kernel, dim3(nblk(3 * ncol)), dim3(256), 0, st, a);
   hipLaunchKernelGGL(adj_remap_finish_kernel, dim3(nblk(ncol)), dim3(256), 0, st, a);
 }
