@@ -30,8 +30,14 @@
  *                     on the sphere, sum(weak laplacian) == 0
  *   DSS               continuity of every node, DSS(spheremp c) rspheremp == c,
  *                     54 ne^2 + 2 unique nodes
- *   CAAR              the steady JW jet stays balanced, converging with ne
+ *   CAAR              the steady JW jet stays balanced, converging with ne;
+ *                     over uniform p and T one stage gives omega p = the
+ *                     pressure tendency of its own thickness change and
+ *                     d ln T = kappa d ln p (potential temperature kept)
+ *   RK5 stages        third-order convergence in time (error / 8 per halving)
  *   hyperviscosity    damping of a Y2 harmonic at dt nu (6/a^2)^2 per subcycle;
+ *                     nu_top sponge: Y2 damped at {4,2,1} dt nu_top 6/a^2 on
+ *                     the top three levels, not below;
  *                     tensor form: identity tensor == simple laplacian bit for
  *                     bit, cartesian vector laplacian leaves rigid rotation
  *                     undamped and agrees with the contravariant form, both to
@@ -542,7 +548,7 @@ static void caar_element(Oracle *o, int ie, int nm1, int n0, int np1, int n0_qdp
     for (int k = 0; k < L; ++k) {
       const size_t i = (size_t)pt * L + k;
       const double vgrad_p = v0n[i] * g0[i] + v1n[i] * g1[i];
-      omb[i] = (vgrad_p - (integration + 0.5 * divv[i])) / p[i];
+      omb[i] = (vgrad_p - (integration + (PLANT(15) ? 1.0 : 0.5) * divv[i])) / p[i]; /* plant 15: the 1/2 dropped */
       integration = integration + divv[i];
     }
   }
@@ -601,7 +607,7 @@ static void caar_element(Oracle *o, int ie, int nm1, int n0, int np1, int n0_qdp
     for (int k = 0; k < L; ++k) {
       const size_t i = (size_t)pt * L + k;
       const double vgrad_t = v0n[i] * b0[i] + v1n[i] * b1[i];
-      const double ttens = (o->rsplit > 0 ? 0 : (PLANT(12) ? tva[i] : -tva[i])) - vgrad_t + KAPPA * tv[i] * omb[i]; /* plant 12 */
+      const double ttens = (o->rsplit > 0 ? 0 : (PLANT(12) ? tva[i] : -tva[i])) - vgrad_t + (PLANT(16) ? -KAPPA : KAPPA) * tv[i] * omb[i]; /* plants 12, 16 */
       double temp = ttens * dt + tnm1[i];
       temp *= sph[pt];
       tnp1[i] = temp;
@@ -690,10 +696,17 @@ static void u3_5stage(Oracle *o, int nm1, int n0, int np1, double dt, double eta
 #pragma omp parallel for schedule(static)
   for (int ie = 0; ie < o->nelem; ++ie)
     for (size_t i = 0; i < E3; ++i) {
+#if ORACLE_PLANT == 17 /* plant 17: stage base (4 u1 - u0)/3 */
+      T(ie, nm1)[i] = (4.0 * T(ie, nm1)[i] - T(ie, n0)[i]) / 3.0;
+      V(ie, nm1, 0)[i] = (4.0 * V(ie, nm1, 0)[i] - V(ie, n0, 0)[i]) / 3.0;
+      V(ie, nm1, 1)[i] = (4.0 * V(ie, nm1, 1)[i] - V(ie, n0, 1)[i]) / 3.0;
+      DP(ie, nm1)[i] = (4.0 * DP(ie, nm1)[i] - DP(ie, n0)[i]) / 3.0;
+#else
       T(ie, nm1)[i] = (5.0 * T(ie, nm1)[i] - T(ie, n0)[i]) / 4.0;
       V(ie, nm1, 0)[i] = (5.0 * V(ie, nm1, 0)[i] - V(ie, n0, 0)[i]) / 4.0;
       V(ie, nm1, 1)[i] = (5.0 * V(ie, nm1, 1)[i] - V(ie, n0, 1)[i]) / 4.0;
       DP(ie, nm1)[i] = (5.0 * DP(ie, nm1)[i] - DP(ie, n0)[i]) / 4.0;
+#endif
     }
   caar_run(o, nm1, np1, np1, 3.0 * dt / 4.0, 3.0 * eta_ave_w / 4.0, work);
 }
@@ -701,7 +714,7 @@ static void u3_5stage(Oracle *o, int nm1, int n0, int np1, double dt, double eta
 /* ======================================================== hyperviscosity */
 static void hv_run(Oracle *o, int np1, double dt_in, double eta_ave_w, double *work) {
   const double dt = dt_in / o->hypervis_subcycle;
-  const double nu_scale_top[3] = {4.0 * o->nu_top, 2.0 * o->nu_top, 1.0 * o->nu_top};
+  const double nu_scale_top[3] = {(PLANT(18) ? -4.0 : 4.0) * o->nu_top, 2.0 * o->nu_top, 1.0 * o->nu_top}; /* plant 18 */
   for (int cyc = 0; cyc < o->hypervis_subcycle; ++cyc) {
     /* first laplacian */
 #pragma omp parallel for schedule(static)

@@ -37,6 +37,19 @@ then fail.
   moist, the surface-pressure mass) equals the initial mass plus the clipped
   dt*FQ, since the step itself conserves mass; a uniform FT raises T by
   dt*FT.  Planted: forcing sign, missing negative-mass clip.
+* omega and the energy conversion (CaarFunctorImpl.hpp:430-463, 854-889):
+  in one CAAR stage over horizontally uniform p and T, omega p equals the
+  pressure tendency built from the stage's own thickness change, and T
+  changes as potential temperature conservation requires
+  (d ln T = kappa d ln p).  Planted: the 1/2 weight of the own level in the
+  omega integral, the sign of kappa T_v omega.
+* RK5 (prim_advance_exp.cpp:113-161): third-order convergence in time of
+  the RK stages alone (error ratio about 8 per halving of dt).  Planted: the
+  stage base (4 u1 - u0)/3 for (5 u1 - u0)/4.
+* nu_top sponge (HyperviscosityFunctorImpl.cpp:24-38): a Y2 temperature
+  harmonic is damped on the top three levels at 4, 2 and 1 times
+  dt nu_top 6/a^2, and not below.  Planted: the sign of the top level's
+  coefficient.
 """
 import numpy as np
 import pytest
@@ -379,3 +392,149 @@ def test_rsplit0_conserves_mass_and_agrees_with_lagrangian(oracle_lib):
 def test_rsplit0_check_detects_planted_vertical_advection_sign(oracle_lib):
     _, _, dT, dv = eulerian_vs_lagrangian(oracle.plant(12))
     assert not (dT < 0.02 and dv < 5e-3)
+
+
+# ------------------------------------- CAAR: omega and the energy conversion
+KAPPA = 287.04 / 1005.0
+
+
+def first_stage_residuals(L, ne=4):
+    """One CAAR stage (the first RK stage, dt/5) from a state whose pressure
+    and temperature are horizontally uniform on every model level (uniform
+    ps, isothermal levels) under a divergent, level-dependent wind, so that
+    v.grad(p) = v.grad(T) = 0 and, at the four interior GLL points of each
+    element (no DSS sharing there), the stage's own thickness change is
+    d(dp_k)/dt = -div(v dp)_k.
+
+    * omega (CaarFunctorImpl.hpp:854-889) is Dp/Dt / p: with
+      p_k = p_top + sum_{j<k} dp_j + dp_k/2 its tendency is
+      dp_k/dt = sum_{j<k} d(dp_j)/dt + d(dp_k)/dt / 2, so omega_k p_k must
+      equal that, as measured from the stage's dp3d output;
+    * the temperature equation (:430-463) conserves potential temperature
+      T (p0/p)^kappa along the (here purely vertical) motion:
+      d ln T / dt = kappa d ln p / dt.
+
+    Returns the largest relative residual of each statement."""
+    nu = 1.0  # hyperviscosity off (the C-ABI needs nu > 0)
+    pr = build_problem(ne=ne, nlev=26, qsize=0, seed=0, nu=nu, nu_p=nu, nu_q=nu, nu_s=nu, nu_div=nu, nu_top=0.0)
+    st, lat, lon = pr.state, pr.mesh.lat, pr.mesh.lon
+    prof = np.linspace(0.5, 1.5, pr.nlev)[None, None, :, None, None]
+    st["v"][:, :, :, 0] = (5.0 * np.cos(lat) * np.sin(lon))[:, None, None] * prof
+    st["v"][:, :, :, 1] = (10.0 * np.sin(2 * lat) * (1 + 0.3 * np.cos(lon)))[:, None, None] * prof
+    st["T"][:] = np.linspace(220.0, 290.0, pr.nlev)[None, None, :, None, None]
+    s = oracle.OracleSession(pr, L)
+    L.oracle_debug_stop(s.h, 1)
+    s.run(1)
+    out = s.state()
+    s.close()
+    h = pr.dt / 5.0
+    nm1, n0 = 0, 1  # the first stage writes time level nm1 from n0
+    dp_old, dp_new = out["dp3d"][:, n0], out["dp3d"][:, nm1]
+    t_old, t_new = out["T"][:, n0], out["T"][:, nm1]
+    omega = 4.0 * out["omega_p"]  # the first stage accumulates omega/p with weight 1/4
+    inner = (slice(None), slice(None), slice(1, 3), slice(1, 3))
+    ddp = (dp_new - dp_old) / h
+    dpdt = (np.cumsum(ddp, axis=1) - ddp) + 0.5 * ddp
+    p = pr.vcoord.hyai[0] * pr.vcoord.ps0 + np.cumsum(dp_old, axis=1) - 0.5 * dp_old
+    r_omega = np.abs(omega * p - dpdt)[inner].max() / np.abs(dpdt[inner]).max()
+    dlnt = ((t_new - t_old) / (h * t_old))[inner]
+    dlnp = KAPPA * (dpdt / p)[inner]
+    r_theta = np.abs(dlnt - dlnp).max() / np.abs(dlnp).max()
+    return float(r_omega), float(r_theta)
+
+
+def test_omega_is_the_pressure_tendency(oracle_lib):
+    r_omega, r_theta = first_stage_residuals(oracle_lib)
+    assert r_omega < 1e-9 and r_theta < 1e-9, (r_omega, r_theta)
+
+
+def test_omega_check_detects_planted_half_weight(oracle_lib):
+    r_omega, _ = first_stage_residuals(oracle.plant(15))
+    assert not r_omega < 1e-9
+
+
+def test_theta_check_detects_planted_energy_conversion_sign(oracle_lib):
+    _, r_theta = first_stage_residuals(oracle.plant(16))
+    assert not r_theta < 1e-9
+
+
+# ---------------------------------------------- RK5: third-order convergence
+def rk_convergence_ratios(L, tend=7200.0, dts=(600.0, 300.0, 150.0), ref=18.75):
+    """Ullrich's five-stage scheme (prim_advance_exp.cpp:113-161) is third
+    order in time: over the same end time, halving dt divides the error by 8.
+    The step is reduced to the RK stages alone: Eulerian vertical coordinate
+    (rsplit = 0, no remap of the states), no tracers, hyperviscosity off, so
+    there is no operator splitting.  The error is the rel L2 distance of
+    (v, T, ps) to a run with dt = 18.75 s.  Returns the error ratios of
+    consecutive halvings."""
+    def run(dt):
+        nu = 1.0
+        pr = build_problem(ne=4, nlev=26, qsize=0, seed=0, dt=dt, nu=nu, nu_p=nu, nu_q=nu, nu_s=nu, nu_div=nu,
+                           nu_top=0.0, rsplit=0, hypervis_subcycle=1)
+        s = oracle.OracleSession(pr, L)
+        nstep, nm1, n0, np1 = s.run(int(round(tend / dt)))
+        out = s.state()
+        s.close()
+        return np.concatenate([out["v"][:, n0].ravel(), out["T"][:, n0].ravel(), out["ps_v"][:, n0].ravel()])
+    xr = run(ref)
+    err = [np.linalg.norm(run(dt) - xr) / np.linalg.norm(xr) for dt in dts]
+    return [err[i] / err[i + 1] for i in range(len(err) - 1)]
+
+
+def test_rk5_is_third_order(oracle_lib):
+    r = rk_convergence_ratios(oracle_lib)
+    assert 6.0 < r[0] < 10.0 and 6.5 < r[1] < 9.5, r
+
+
+def test_rk5_check_detects_planted_stage_base(oracle_lib):
+    r = rk_convergence_ratios(oracle.plant(17))
+    assert not 6.5 < r[1] < 9.5, r
+
+
+# ------------------------------------------------ hyperviscosity: nu_top sponge
+def sponge_damping(L, ne=4):
+    """The nu_top sponge (HyperviscosityFunctorImpl.cpp:24-38) adds
+    nu_top * {4, 2, 1} times the weak laplacian on the top three levels.  A
+    zonal l=2 temperature harmonic Y is an eigenfunction of the laplacian
+    (eigenvalue -6/a^2), so over one step of hyperviscosity (3 subcycles) it
+    is damped by dt * scale_k * nu_top * 6/a^2 * Y on level k, and not at
+    all below; the biharmonic itself is switched off (nu = 1).  Returns the
+    measured damping on levels 0..3 in units of dt nu_top 6/a^2."""
+    nu, nu_top, sub, dt = 1.0, 2.5e5, 3, 300.0
+    pr = build_problem(ne=ne, nlev=26, qsize=1, seed=0, dt=dt, nu=nu, nu_p=nu, nu_q=nu, nu_s=nu, nu_div=nu,
+                       nu_top=nu_top, hypervis_subcycle=sub)
+    st = pr.state
+    x = np.sin(pr.mesh.lat)
+    y2 = 0.5 * (3 * x * x - 1)
+    st["v"][:] = 0.0
+    st["phis"][:] = 0.0
+    st["T"][:] = 280.0 + 5.0 * y2[:, None, None]
+    st["Qdp"][:] = 0.3 * st["dp3d"][:, :2, None]
+    res = []
+    for stop in (2, 3):  # after the RK stages, after hyperviscosity
+        s = oracle.OracleSession(pr, L)
+        L.oracle_debug_stop(s.h, stop)
+        s.run(1)
+        res.append(s.state()["T"][:, 2])  # time level np1
+        s.close()
+    w = pr.metric["spheremp"]
+    out = []
+    for k in range(4):
+        tb, d = res[0][:, k], res[1][:, k] - res[0][:, k]
+        amp = (w * (tb - tb.mean()) * y2).sum() / (w * y2 * y2).sum()
+        damp = (w * d * y2).sum() / (w * y2 * y2).sum()
+        out.append(-damp / (amp * dt * nu_top * 6.0 / A ** 2))
+    return out
+
+
+def sponge_ok(r):
+    return bool(abs(r[0] - 4) < 4e-3 and abs(r[1] - 2) < 2e-3 and abs(r[2] - 1) < 1e-3 and abs(r[3]) < 1e-6)
+
+
+def test_nu_top_sponge_damps_the_top_levels_4_2_1(oracle_lib):
+    r = sponge_damping(oracle_lib)
+    assert sponge_ok(r), r
+
+
+def test_sponge_check_detects_planted_sign(oracle_lib):
+    assert not sponge_ok(sponge_damping(oracle.plant(18)))
