@@ -3,11 +3,13 @@ the C-ABI.
 
 The forward pass runs ``prim_run_subcycle_c`` (the HIP step) and keeps the
 prognostic state (v, T, ps) every ``every`` steps as device checkpoints.  The
-backward pass replays each segment from its checkpoint, keeping the input
-state of every step of the segment, and runs ``hommexx_hip_adjoint_step``
-(hand-written HIP adjoint kernels of CAAR, hyperviscosity, DSS and the
-PPM remap) over the
-segment's steps in reverse.  Device memory: ``ceil(nsteps/every)``
+backward pass replays each segment from its checkpoint with
+``hommexx_hip_dynamics_step`` (the same map without the tracer step, which a
+dry run's dynamics do not see), keeping the input state of every step of the
+segment, and runs ``hommexx_hip_adjoint_step`` (hand-written HIP adjoint
+kernels of CAAR, hyperviscosity, DSS and the PPM remap) over the segment's
+steps in reverse.  Neither call touches the session's model state, which
+stays at the forward pass's end.  Device memory: ``ceil(nsteps/every)``
 checkpoints plus ``every`` step states, each (v, T, ps).
 
 The adjoint covers dry configurations with any ``rsplit`` and ``qsplit``: one
@@ -54,6 +56,7 @@ class DiffSession:
         L.hommexx_hip_get_state.argtypes = [_P, _P, _P]
         L.hommexx_hip_set_state.argtypes = [_P, _P, _P]
         L.hommexx_hip_adjoint_step.argtypes = [_D, _P, _P, _P, _P, _P, _P]
+        L.hommexx_hip_dynamics_step.argtypes = [_D, _P, _P, _P]
         self.L = L
         E, nl = prob.nelem, prob.nlev
         self.shapes = ((E, nl, 2, 4, 4), (E, nl, 4, 4), (E, 4, 4))
@@ -76,6 +79,15 @@ class DiffSession:
 
     def step(self, nsteps=1):
         self.h.run(nsteps)
+
+    def dynamics_step(self, state):
+        """(v, T, ps) -> its image under one step's dynamics (and remap), in
+        place; the session's model state is not changed"""
+        torch.cuda.synchronize(self.device)
+        rc = self.L.hommexx_hip_dynamics_step(self.prob.dt, *[_ptr(t) for t in state])
+        if rc:
+            self.h._check("hommexx_hip_dynamics_step")
+            raise HommexxError("hommexx_hip_dynamics_step failed (%d)" % rc)
 
     def adjoint_step(self, state, lam):
         """lam (adjoint of the step's output) -> adjoint of its input, in place"""
@@ -118,11 +130,11 @@ class PreqxTrajectory(torch.autograd.Function):
         for s in reversed(range(len(ctx.ckpts))):
             n0 = s * every
             n1 = min(n0 + every, nsteps)
-            sess.set_state(*ctx.ckpts[s])
             states = [ctx.ckpts[s]]
             for _ in range(n0, n1 - 1):
-                sess.step()
-                states.append(sess.get_state())
+                nxt = tuple(t.clone() for t in states[-1])
+                sess.dynamics_step(nxt)
+                states.append(nxt)
             for st in reversed(states):
                 sess.adjoint_step(st, lam)
         ctx.ckpts = None

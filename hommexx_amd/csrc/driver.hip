@@ -24,6 +24,7 @@
 
 using namespace hx;
 
+
 namespace {
 
 struct Params {
@@ -44,7 +45,8 @@ struct Timer {
 struct Ctx {
   // configuration
   bool session = false;
-  int nlev_cfg = 72, qsize_d_cfg = -1;
+  // the reference's build-time PLEV and QSIZE_D (Makefile PREQX_PLEV, QSIZE_D)
+  int nlev_cfg = HOMMEXX_PREQX_PLEV, qsize_d_cfg = HOMMEXX_QSIZE_D;
   bool abort_on_error = true;
   int last_error = 0;
   std::string msg;
@@ -72,6 +74,7 @@ struct Ctx {
   // states
   double *v[3] = {}, *t[3] = {}, *dp[3] = {}, *ps[3] = {}, *qdp[2] = {}, *Q = nullptr;
   std::vector<double> qdp_tail;  // Qdp of tracers qsize..qsize_d-1 as received (never evolved)
+  std::vector<double> fq_tail;   // FQ of tracers qsize..qsize_d-1 as received (never used)
   // derived
   double *phi = nullptr, *omega = nullptr, *eta_dot = nullptr, *dpdiss_ave = nullptr, *dpdiss_bih = nullptr,
          *derived_dp = nullptr, *divdp = nullptr, *divdp_proj = nullptr, *vn0 = nullptr;
@@ -301,8 +304,10 @@ void alloc_work() {
   if (c.be_set) return;
   const size_t e3 = (size_t)c.nelem * c.nlev * 16;
   const size_t q3 = e3 * (c.qsize ? c.qsize : 1);
+  // pre-exchange buffers: whole 8-level chunks per element and field
+  const size_t e3c = (size_t)c.nelem * lc_field(c.nlev), q3c = e3c * (c.qsize ? c.qsize : 1);
   c.cv = dalloc<double>(2 * e3);
-  c.ct = dalloc<double>(e3);
+  c.ct = dalloc<double>(e3c > e3 ? e3c : e3);  // also the stage-2 fdss_pre
   c.cdp = dalloc<double>(e3);
   c.cdiv = dalloc<double>(e3);
   c.vt_a = dalloc<double>(2 * e3);
@@ -311,13 +316,19 @@ void alloc_work() {
   c.tt_b = dalloc<double>(e3);
   c.dpt_a = dalloc<double>(e3);
   c.dpt_b = dalloc<double>(e3);
-  c.qdp_pre = dalloc<double>(q3);
-  c.qbih = dalloc<double>(q3);
-  c.qbih_pre = dalloc<double>(q3);
+  // Three 40-F3 tracer buffers share two allocations (at Q = 40):
+  //   qdp_pre : stage-1 pre-exchange, consumed by the stage-2 gather; then
+  //             the biharmonic result qbih (bih_post -> stage-3 advection)
+  //   qbih_pre: stage-2 pre-exchange (-> euler_dss), then the first
+  //             laplacian (qmm_lap -> bih_post), then the stage-3
+  //             pre-exchange (-> euler_dss)
+  c.qdp_pre = dalloc<double>(q3c > q3 ? q3c : q3);
+  c.qbih = c.qdp_pre;
+  c.qbih_pre = dalloc<double>(q3c);
   c.qdp_pre2 = c.qbih_pre;
   c.qlim[0] = dalloc<double>((size_t)c.nelem * (c.qsize ? c.qsize : 1) * 2 * c.nlev);
   c.qlim[1] = dalloc<double>((size_t)c.nelem * (c.qsize ? c.qsize : 1) * 2 * c.nlev);
-  c.fdss_pre = dalloc<double>(e3);
+  c.fdss_pre = dalloc<double>(e3c);
   c.fdss_pre2 = c.ct;
   c.err_d = dalloc<int>(1);
   if (!c.err_h) hipHostMalloc((void **)&c.err_h, sizeof(int));
@@ -521,6 +532,8 @@ void advec_tracers(double dt) {
     a.rhs_multiplier = (double)stage;
     a.rhs_viss = stage == 2 ? 3.0 : 0.0;
     a.fdss = stage == 0 ? c.divdp_proj : stage == 1 ? c.eta_dot : c.omega;
+    // stage 3 reads qbih, which occupies the stage-1 buffer
+    a.qdp_pre = stage == 2 ? c.qbih_pre : c.qdp_pre;
     a.qdp_avg_n0 = stage == 2 ? c.qdp[c.n0_qdp] : nullptr;
     // stage 2 gathers stage 1's exchange itself (the assembled stage-1 Qdp
     // is never stored) unless the caller stops after stage 1
@@ -642,18 +655,20 @@ StateBuf carve_state(double *&q, size_t f3) {
   return s;
 }
 
-bool adj_alloc(int nd) {
+// doubles of the adjoint step's buffers for nd dynamics steps per call
+size_t adj_total_doubles(int nd) {
+  Ctx &c = C();
+  const size_t f3 = (size_t)c.nelem * c.nlev * 16;
+  const int nstates = 5 + 1 + (c.p.hypervis_subcycle + 1) + 6 + nd;  // U, W, H, LX L0 L1 LP S M, X
+  return (size_t)nstates * 4 * f3 + 12 * f3 + (size_t)c.nelem * 16 + adj_remap_scratch(c.nelem, c.nlev);
+}
+
+// carve the adjoint step's buffers from base (adj_total_doubles(nd) doubles)
+void adj_layout(AdjBufs &b, double *base, int nd) {
   Ctx &c = C();
   const size_t f3 = (size_t)c.nelem * c.nlev * 16;
   const int nsub = c.p.hypervis_subcycle;
-  if (adjb && adjb->f3 == f3 && adjb->nsub == nsub && adjb->nd == nd) return true;
-  if (!adjb) adjb = new AdjBufs();
-  AdjBufs &b = *adjb;
-  const int nstates = 5 + 1 + (nsub + 1) + 6 + nd;  // U, W, H, LX L0 L1 LP S M, X
-  const size_t nrs = adj_remap_scratch(c.nelem, c.nlev);
-  const size_t total = (size_t)nstates * 4 * f3 + 12 * f3 + (size_t)c.nelem * 16 + nrs;
-  b.base = dalloc<double>(total);  // owned by the session (freed with it)
-  if (!b.base) return false;
+  b.base = base;
   b.f3 = f3;
   b.nsub = nsub;
   b.nd = nd;
@@ -681,6 +696,16 @@ bool adj_alloc(int nd) {
   b.stot = q;
   q += (size_t)c.nelem * 16;
   b.rscr = q;
+}
+
+bool adj_alloc(int nd) {
+  Ctx &c = C();
+  const size_t f3 = (size_t)c.nelem * c.nlev * 16;
+  if (adjb && adjb->f3 == f3 && adjb->nsub == c.p.hypervis_subcycle && adjb->nd == nd) return true;
+  if (!adjb) adjb = new AdjBufs();
+  double *base = dalloc<double>(adj_total_doubles(nd));  // owned by the session (freed with it)
+  if (!base) return false;
+  adj_layout(*adjb, base, nd);
   return true;
 }
 
@@ -827,6 +852,22 @@ void push_tracers(double *f90, const double *dev, int ntl_f, int tl) {
              e3 * sizeof(double));
 }
 
+// Tracers::push_qdp (Tracers.cpp:46-51): both Qdp time levels of every tracer
+// slot into the Fortran array (np,np,nlev,qsize_d,2,nelemd); the slots beyond
+// qsize go back as they were received at init.
+void push_qdp(double *Qdp) {
+  Ctx &c = C();
+  const size_t e3 = (size_t)c.nlev * 16;
+  for (int tl = 0; tl < 2; ++tl) push_tracers(Qdp, c.qdp[tl], 2, tl);
+  if (c.qsize_d > c.qsize && !c.qdp_tail.empty()) {
+    const size_t tail = (size_t)(c.qsize_d - c.qsize) * e3;
+    for (int ie = 0; ie < c.nelem; ++ie)
+      for (int tl = 0; tl < 2; ++tl)
+        memcpy(Qdp + (((size_t)ie * 2 + tl) * c.qsize_d + c.qsize) * e3, &c.qdp_tail[((size_t)ie * 2 + tl) * tail],
+               tail * sizeof(double));
+  }
+}
+
 // CAM forcing buffers FT, FM, FQ, allocated (zeroed, as the reference's
 // views are) the first time a forcing run or a forcing push needs them
 bool ensure_forcing() {
@@ -916,6 +957,114 @@ void diag_energy(bool before_advance, int ivar) {
   diag_copy(c.dg[6], 4, ivar, 16, out + 2 * n, 16);  // kener
   diag_copy(c.dg[7], 4, ivar, 16, out + 3 * n, 16);  // pener
   hipok(hipStreamSynchronize(c.st), "prim_energy_halftimes");
+}
+
+// One dynamics step of the differentiable map from X[i]: RK5 stage inputs
+// U0..U4, w and the HV cycle states H0..Hnsub (H[nsub] is the step's output),
+// with the kernels of the forward step (no accumulators for the tracers).
+void adj_dyn_forward(AdjBufs &b, int i, double dt) {
+  Ctx &c = C();
+  const size_t f3 = b.f3;
+  const int nsub = c.p.hypervis_subcycle;
+  std::vector<StateBuf> &U = b.U, &H = b.H, &X = b.X;
+  copy_state(U[0], X[i], f3);
+  caar_run(U[0], U[0], U[1], dt / 5.0, 0.0);
+  caar_run(U[0], U[1], U[2], dt / 5.0, 0.0);
+  caar_run(U[0], U[2], U[3], dt / 3.0, 0.0);
+  caar_run(U[0], U[3], U[4], 2.0 * dt / 3.0, 0.0);
+  copy_state(b.W, U[1], f3);
+  launch_rk_combo(c.st, c.nelem, c.nlev, b.W.v, b.W.t, b.W.dp, U[0].v, U[0].t, U[0].dp);
+  caar_run(b.W, U[4], H[0], 3.0 * dt / 4.0, 0.0);
+  for (int cyc = 0; cyc < nsub; ++cyc) {
+    copy_state(H[cyc + 1], H[cyc], f3);
+    launch_hv_cycle(c.st, hv_args(H[cyc + 1], dt, 0.0));
+  }
+}
+
+// X[0] from (v, T, ps), then the nd dynamics steps of one
+// prim_run_subcycle_c call; H[nsub] holds the last step's output
+void adj_forward_steps(AdjBufs &b, int nd, double dt, const double *v, const double *T, const double *ps) {
+  Ctx &c = C();
+  const size_t f3 = b.f3;
+  hipMemcpyAsync(b.X[0].v, v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  hipMemcpyAsync(b.X[0].t, T, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  launch_dp3d_from_ps(c.st, c.nelem, c.nlev, c.hyai_d, c.hybi_d, c.ps0, ps, b.X[0].dp);
+  for (int i = 0; i < nd; ++i) {
+    adj_dyn_forward(b, i, dt);
+    if (i + 1 < nd) copy_state(b.X[i + 1], b.H[c.p.hypervis_subcycle], f3);
+  }
+}
+
+// the configurations the differentiable form covers
+bool adj_supported(const char *who) {
+  Ctx &c = C();
+  if (!ready(who)) return false;
+  if (c.p.moist || !c.consthv || c.form != 0 || c.p.ftype != -1) {
+    fail(70, "%s: needs a dry run, constant-coefficient hyperviscosity, no forcing and the FP64 form", who);
+    return false;
+  }
+  if (!c.be_set) init_boundary_exchanges_c();
+  return true;
+}
+
+// The adjoint step on the buffers b (adj_layout for nd dynamics steps)
+int adj_step_run(AdjBufs &b, int nd, double dt, const double *v, const double *T, const double *ps, double *lam_v,
+                 double *lam_T, double *lam_ps, const char *who) {
+  Ctx &c = C();
+  const size_t f3 = b.f3;
+  const int nsub = c.p.hypervis_subcycle;
+  std::vector<StateBuf> &U = b.U, &H = b.H;
+  auto dyn_forward = [&](int i) { adj_dyn_forward(b, i, dt); };
+  // forward: the input of every dynamics step; the last step's stage states
+  // stay in U, W, H for the first reverse step
+  {
+    Scope s("adj_forward");
+    adj_forward_steps(b, nd, dt, v, T, ps);
+  }
+  // reverse sweep
+  Scope s("adj_reverse");
+  hipMemcpyAsync(b.LX.v, lam_v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  hipMemcpyAsync(b.LX.t, lam_T, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  if (c.p.rsplit == 0) {
+    launch_adj_fill_dp(c.st, c.nelem, c.nlev, lam_ps, b.LX.dp);  // ps = hyai(0) ps0 + sum_k dp3d
+  } else {
+    // linearised PPM remap of v and T from the thicknesses of the last step
+    AdjRemapArgs r{};
+    r.nelem = c.nelem;
+    r.nlev = c.nlev;
+    r.alg = c.p.remap_alg;
+    r.v = H[nsub].v;
+    r.t = H[nsub].t;
+    r.dp = H[nsub].dp;
+    r.l_v = b.LX.v;
+    r.l_t = b.LX.t;
+    r.l_dp = b.LX.dp;
+    r.l_ps = lam_ps;
+    r.hyai = c.hyai_d;
+    r.hybi = c.hybi_d;
+    r.ps0 = c.ps0;
+    r.scr = b.rscr;
+    launch_adj_remap(c.st, r);
+  }
+  for (int i = nd - 1; i >= 0; --i) {
+    if (i < nd - 1) {
+      dyn_forward(i);  // this step's stage states
+      copy_state(b.LX, b.L0, f3);  // adjoint of its output = adjoint of the next step's input
+    }
+    for (int cyc = nsub - 1; cyc >= 0; --cyc) hv_adjoint(H[cyc], H[cyc + 1], b.LX, dt / nsub);
+    launch_adj_zero(c.st, 4 * f3, b.L0.v);
+    launch_adj_zero(c.st, 4 * f3, b.L1.v);
+    // stage 5 from w = (5 u1 - u0)/4 and u4; then stages 4..1
+    caar_adjoint(U[4], b.LX, 3.0 * dt / 4.0, b.LX, false, b.L1, 1.25, &b.L0, -0.25);
+    caar_adjoint(U[3], b.LX, 2.0 * dt / 3.0, b.LX, false, b.L0, 1.0, nullptr, 0.0);
+    caar_adjoint(U[2], b.LX, dt / 3.0, b.LX, false, b.L0, 1.0, nullptr, 0.0);
+    caar_adjoint(U[1], b.LX, dt / 5.0, b.L1, true, b.L0, 1.0, nullptr, 0.0);
+    caar_adjoint(U[0], b.L1, dt / 5.0, b.L0, true, b.L0, 1.0, nullptr, 0.0);
+  }
+  hipMemcpyAsync(lam_v, b.L0.v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  hipMemcpyAsync(lam_T, b.L0.t, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  launch_adj_ps(c.st, c.nelem, c.nlev, c.hybi_d, b.L0.dp, lam_ps);  // dp3d = dhyai ps0 + dhybi ps
+  return hipok(hipStreamSynchronize(c.st), who) ? 0 : 4;
 }
 
 }  // namespace
@@ -1329,14 +1478,7 @@ void cxx_push_results_to_f90(double *&v, double *&T, double *&dp3d, double *&Qdp
     push_levels(dp3d, c.dp[tl], 3, tl, e3);
     push_levels(ps_v, c.ps[tl], 3, tl, 16);
   }
-  for (int tl = 0; tl < 2; ++tl) push_tracers(Qdp, c.qdp[tl], 2, tl);
-  if (c.qsize_d > c.qsize && !c.qdp_tail.empty()) {
-    const size_t tail = (size_t)(c.qsize_d - c.qsize) * e3;
-    for (int ie = 0; ie < c.nelem; ++ie)
-      for (int tl = 0; tl < 2; ++tl)
-        memcpy(Qdp + (((size_t)ie * 2 + tl) * c.qsize_d + c.qsize) * e3, &c.qdp_tail[((size_t)ie * 2 + tl) * tail],
-               tail * sizeof(double));
-  }
+  push_qdp(Qdp);
   push_tracers(Q, c.Q, 1, 0);
   for (int ie = 0; ie < c.nelem; ++ie)
     for (int q = c.qsize; q < c.qsize_d; ++q) memset(Q + ((size_t)ie * c.qsize_d + q) * e3, 0, e3 * sizeof(double));
@@ -1351,10 +1493,22 @@ void cxx_push_forcing_to_f90(double *FM, double *FT, double *FQ) {
   const size_t e3 = (size_t)c.nlev * 16;
   push_levels(FM, c.fm, 1, 0, 2 * e3);
   push_levels(FT, c.ft, 1, 0, e3);
-  if (c.p.ftype == 0 && FQ) push_tracers(FQ, c.fq, 1, 0);
+  if (c.p.ftype == 0 && FQ) {
+    push_tracers(FQ, c.fq, 1, 0);
+    // slots beyond qsize: what the last f90_push_forcing_to_cxx sent (zero
+    // before any), as the reference's QSIZE_D-wide fq view holds them
+    const size_t tail = (size_t)(c.qsize_d - c.qsize) * e3;
+    for (int ie = 0; tail && ie < c.nelem; ++ie) {
+      double *dst = FQ + ((size_t)ie * c.qsize_d + c.qsize) * e3;
+      if (c.fq_tail.empty()) memset(dst, 0, tail * sizeof(double));
+      else memcpy(dst, &c.fq_tail[(size_t)ie * tail], tail * sizeof(double));
+    }
+  }
 }
 
-// cxx_f90_interface.cpp:180-205
+// cxx_f90_interface.cpp:180-205.  FM, FT (and FQ with ftype 0) go to the
+// device; Qdp goes the other way, as the reference's tracers.push_qdp does
+// (:201): the Fortran array receives the device's Qdp.
 void f90_push_forcing_to_cxx(double *FM, double *FT, double *FQ, double *Qdp) {
   Ctx &c = C();
   if (!ensure_forcing()) return;
@@ -1362,14 +1516,21 @@ void f90_push_forcing_to_cxx(double *FM, double *FT, double *FQ, double *Qdp) {
   hipStreamSynchronize(c.st);
   pull_levels(c.fm, FM, 1, 0, 2 * e3);
   pull_levels(c.ft, FT, 1, 0, e3);
-  if (c.p.ftype == 0 && FQ) pull_tracers(c.fq, FQ, 1, 0);
-  for (int tl = 0; tl < 2; ++tl) pull_tracers(c.qdp[tl], Qdp, 2, tl);
+  if (c.p.ftype == 0 && FQ) {
+    pull_tracers(c.fq, FQ, 1, 0);
+    const size_t tail = (size_t)(c.qsize_d - c.qsize) * e3;
+    c.fq_tail.resize((size_t)c.nelem * tail);
+    for (int ie = 0; tail && ie < c.nelem; ++ie)
+      memcpy(&c.fq_tail[(size_t)ie * tail], FQ + ((size_t)ie * c.qsize_d + c.qsize) * e3, tail * sizeof(double));
+  }
+  if (Qdp && c.states_set) push_qdp(Qdp);
 }
 
 // ============================================================ extensions
 // Dimensions the reference fixes at compile time (PLEV, QSIZE_D); call before
-// init_elements_2d_c.  Defaults: nlev = 72, qsize_d = qsize.  nlev is at most
-// REMAP_MAX_NLEV: the remap keeps each element's target table in LDS.
+// init_elements_2d_c.  Defaults: the build's PREQX_PLEV and QSIZE_D; a
+// qsize_d of 0 or less means qsize.  nlev is at most REMAP_MAX_NLEV: the
+// remap keeps each element's target table in LDS.
 int hommexx_hip_configure(int nlev, int qsize_d) {
   Ctx &c = C();
   if (c.elems_set) return 1;
@@ -1377,6 +1538,12 @@ int hommexx_hip_configure(int nlev, int qsize_d) {
   c.nlev_cfg = nlev;
   c.qsize_d_cfg = qsize_d;
   return 0;
+}
+
+void hommexx_hip_dimensions(int *nlev, int *qsize_d) {
+  Ctx &c = C();
+  *nlev = c.nlev_cfg;
+  *qsize_d = c.qsize_d_cfg;
 }
 
 void hommexx_hip_set_abort_on_error(int on) { C().abort_on_error = on != 0; }
@@ -1458,6 +1625,33 @@ int hommexx_hip_get_state(double *v, double *T, double *ps) {
   return hipok(hipStreamSynchronize(c.st), "hommexx_hip_get_state") ? 0 : 2;
 }
 
+// One field of the current state into device memory dst, library layout:
+// which 0 = v [ie][lev][2][pt], 1 = T, 2 = dp3d [ie][lev][pt], 3 = ps_v
+// [ie][pt] (time level n0), 4 = Qdp of tracer q [ie][lev][pt] (the tracer
+// level the next step starts from, (nstep / qsplit) % 2), 5 = Q of tracer q.
+int hommexx_hip_get_field(int which, int q, double *dst) {
+  Ctx &c = C();
+  if (!c.states_set) {
+    fail(55, "hommexx_hip_get_field: no state");
+    return 1;
+  }
+  if (which < 0 || which > 5 || (which >= 4 && (q < 0 || q >= c.qsize))) {
+    fail(56, "hommexx_hip_get_field: no field %d (tracer %d)", which, q);
+    return 1;
+  }
+  const size_t f3 = (size_t)c.nelem * c.nlev * 16, e3 = (size_t)c.nlev * 16;
+  if (which <= 3) {
+    const double *srcs[4] = {c.v[c.n0], c.t[c.n0], c.dp[c.n0], c.ps[c.n0]};
+    const size_t n[4] = {2 * f3, f3, f3, (size_t)c.nelem * 16};
+    hipMemcpyAsync(dst, srcs[which], n[which] * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  } else {
+    const double *base = which == 4 ? c.qdp[(c.nstep / c.p.qsplit) % 2] : c.Q;
+    hipMemcpy2DAsync(dst, e3 * sizeof(double), base + (size_t)q * e3, (size_t)c.qsize * e3 * sizeof(double),
+                     e3 * sizeof(double), c.nelem, hipMemcpyDeviceToDevice, c.st);
+  }
+  return hipok(hipStreamSynchronize(c.st), "hommexx_hip_get_field") ? 0 : 2;
+}
+
 int hommexx_hip_set_state(const double *v, const double *T, const double *ps) {
   Ctx &c = C();
   if (!c.states_set || !c.hv_set) {
@@ -1473,101 +1667,70 @@ int hommexx_hip_set_state(const double *v, const double *T, const double *ps) {
 }
 
 // Adjoint of one step of length dt (one prim_run_subcycle_c call) of a dry
-// configuration with rsplit = 0 or 1, taken at the step's input state (v, T, ps):
+// configuration (any rsplit and qsplit; constant-coefficient hyperviscosity;
+// the FP64 form), taken at the step's input state (v, T, ps):
 // on entry lam_* hold the adjoint of the step's output (v, T, ps), on return
 // the adjoint of its input.  All pointers are device memory in the layouts of
 // hommexx_hip_get_state.  The model state of the session is not changed.
 int hommexx_hip_adjoint_step(double dt, const double *v, const double *T, const double *ps, double *lam_v,
                              double *lam_T, double *lam_ps) {
   Ctx &c = C();
-  if (!ready("hommexx_hip_adjoint_step")) return 1;
-  if (c.p.moist || !c.consthv || c.form != 0 || c.p.ftype != -1) {
-    fail(70, "hommexx_hip_adjoint_step: needs a dry run, constant-coefficient hyperviscosity, no forcing and the "
-             "FP64 form");
-    return 2;
-  }
-  if (!c.be_set) init_boundary_exchanges_c();
+  if (!adj_supported("hommexx_hip_adjoint_step")) return 2;
   // prim_run_subcycle_c: qsplit dynamics steps per prim_step, rsplit (at
   // least one) prim_steps, then the remap (prim_driver.cpp:115-130,
   // prim_step.cpp:57-80)
   const int nd = c.p.qsplit * (c.p.rsplit > 0 ? c.p.rsplit : 1);
   if (!adj_alloc(nd)) return 3;
+  return adj_step_run(*adjb, nd, dt, v, T, ps, lam_v, lam_T, lam_ps, "hommexx_hip_adjoint_step");
+}
+
+
+
+
+// The map hommexx_hip_adjoint_step differentiates, forward: (v, T, ps) is
+// advanced in place by the dynamics steps of one prim_run_subcycle_c call and
+// the remap of v and T (rsplit > 0), with no tracer step; for a dry run the
+// result equals the (v, T, ps) that prim_run_subcycle_c produces.  The
+// session's model state is not changed.
+int hommexx_hip_dynamics_step(double dt, double *v, double *T, double *ps) {
+  Ctx &c = C();
+  if (!adj_supported("hommexx_hip_dynamics_step")) return 2;
+  const int nd = c.p.qsplit * (c.p.rsplit > 0 ? c.p.rsplit : 1);
+  if (!adj_alloc(nd)) return 3;
   AdjBufs &b = *adjb;
   const size_t f3 = b.f3;
-  const int nsub = c.p.hypervis_subcycle;
-  std::vector<StateBuf> &U = b.U, &H = b.H, &X = b.X;
-  // one dynamics step from X[i]: RK5 stage inputs U0..U4, w and the HV cycle
-  // states H0..Hnsub (H[nsub] is the step's output)
-  auto dyn_forward = [&](int i) {
-    copy_state(U[0], X[i], f3);
-    caar_run(U[0], U[0], U[1], dt / 5.0, 0.0);
-    caar_run(U[0], U[1], U[2], dt / 5.0, 0.0);
-    caar_run(U[0], U[2], U[3], dt / 3.0, 0.0);
-    caar_run(U[0], U[3], U[4], 2.0 * dt / 3.0, 0.0);
-    copy_state(b.W, U[1], f3);
-    launch_rk_combo(c.st, c.nelem, c.nlev, b.W.v, b.W.t, b.W.dp, U[0].v, U[0].t, U[0].dp);
-    caar_run(b.W, U[4], H[0], 3.0 * dt / 4.0, 0.0);
-    for (int cyc = 0; cyc < nsub; ++cyc) {
-      copy_state(H[cyc + 1], H[cyc], f3);
-      launch_hv_cycle(c.st, hv_args(H[cyc + 1], dt, 0.0));
-    }
-  };
-  // forward: the input of every dynamics step; the last step's stage states
-  // stay in U, W, H for the first reverse step
-  {
-    Scope s("adj_forward");
-    hipMemcpyAsync(X[0].v, v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
-    hipMemcpyAsync(X[0].t, T, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
-    launch_dp3d_from_ps(c.st, c.nelem, c.nlev, c.hyai_d, c.hybi_d, c.ps0, ps, X[0].dp);
-    for (int i = 0; i < nd; ++i) {
-      dyn_forward(i);
-      if (i + 1 < nd) copy_state(X[i + 1], H[nsub], f3);
-    }
-  }
-  // reverse sweep
-  Scope s("adj_reverse");
-  hipMemcpyAsync(b.LX.v, lam_v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
-  hipMemcpyAsync(b.LX.t, lam_T, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
-  if (c.p.rsplit == 0) {
-    launch_adj_fill_dp(c.st, c.nelem, c.nlev, lam_ps, b.LX.dp);  // ps = hyai(0) ps0 + sum_k dp3d
+  launch_zero_int(c.st, c.err_d);
+  adj_forward_steps(b, nd, dt, v, T, ps);
+  const StateBuf &o = b.H[c.p.hypervis_subcycle];
+  if (c.p.rsplit > 0) {
+    RemapArgs a{};
+    a.nelem = c.nelem;
+    a.nlev = c.nlev;
+    a.qsize = 0;
+    a.nstates = 3;
+    a.remap_alg = c.p.remap_alg;
+    a.v = o.v;
+    a.t = o.t;
+    a.dp = o.dp;
+    a.ps = ps;
+    a.hyai = c.hyai_d;
+    a.hybi = c.hybi_d;
+    a.ps0 = c.ps0;
+    a.hyai0 = c.hyai0;
+    a.error = c.err_d;
+    launch_remap(c.st, a);
   } else {
-    // linearised PPM remap of v and T from the thicknesses of the last step
-    AdjRemapArgs r{};
-    r.nelem = c.nelem;
-    r.nlev = c.nlev;
-    r.alg = c.p.remap_alg;
-    r.v = H[nsub].v;
-    r.t = H[nsub].t;
-    r.dp = H[nsub].dp;
-    r.l_v = b.LX.v;
-    r.l_t = b.LX.t;
-    r.l_dp = b.LX.dp;
-    r.l_ps = lam_ps;
-    r.hyai = c.hyai_d;
-    r.hybi = c.hybi_d;
-    r.ps0 = c.ps0;
-    r.scr = b.rscr;
-    launch_adj_remap(c.st, r);
+    launch_ps_from_dp(c.st, c.nelem, c.nlev, c.hyai0, c.ps0, o.dp, ps);
   }
-  for (int i = nd - 1; i >= 0; --i) {
-    if (i < nd - 1) {
-      dyn_forward(i);  // this step's stage states
-      copy_state(b.LX, b.L0, f3);  // adjoint of its output = adjoint of the next step's input
-    }
-    for (int cyc = nsub - 1; cyc >= 0; --cyc) hv_adjoint(H[cyc], H[cyc + 1], b.LX, dt / nsub);
-    launch_adj_zero(c.st, 4 * f3, b.L0.v);
-    launch_adj_zero(c.st, 4 * f3, b.L1.v);
-    // stage 5 from w = (5 u1 - u0)/4 and u4; then stages 4..1
-    caar_adjoint(U[4], b.LX, 3.0 * dt / 4.0, b.LX, false, b.L1, 1.25, &b.L0, -0.25);
-    caar_adjoint(U[3], b.LX, 2.0 * dt / 3.0, b.LX, false, b.L0, 1.0, nullptr, 0.0);
-    caar_adjoint(U[2], b.LX, dt / 3.0, b.LX, false, b.L0, 1.0, nullptr, 0.0);
-    caar_adjoint(U[1], b.LX, dt / 5.0, b.L1, true, b.L0, 1.0, nullptr, 0.0);
-    caar_adjoint(U[0], b.L1, dt / 5.0, b.L0, true, b.L0, 1.0, nullptr, 0.0);
+  hipMemcpyAsync(v, o.v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  hipMemcpyAsync(T, o.t, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+  hipMemcpyAsync(c.err_h, c.err_d, sizeof(int), hipMemcpyDeviceToHost, c.st);
+  if (!hipok(hipStreamSynchronize(c.st), "hommexx_hip_dynamics_step")) return 4;
+  if (*c.err_h & 1) {
+    fail(60, "Negative (or nan) layer thickness detected, aborting!");
+    return 5;
   }
-  hipMemcpyAsync(lam_v, b.L0.v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
-  hipMemcpyAsync(lam_T, b.L0.t, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
-  launch_adj_ps(c.st, c.nelem, c.nlev, c.hybi_d, b.L0.dp, lam_ps);  // dp3d = dhyai ps0 + dhybi ps
-  return hipok(hipStreamSynchronize(c.st), "hommexx_hip_adjoint_step") ? 0 : 4;
+  return 0;
 }
 
 // Current time-level indices (0-based) and sizes.

@@ -104,15 +104,15 @@ __global__ __launch_bounds__(256) void euler_nbr_minmax_kernel(EulerArgs a) {
   a.qlim_new[own + L] = mx;
 }
 
-// gather of point (ie, pt) at level k from a levels-fastest field with
-// element stride es (own value, then the sources in the reference's order);
-// gather_lf_load issues the four loads, gather_sum adds them
-// (FP32 buffers of the mixed form are accumulated in FP64 the same way)
+// gather of point (ie, pt) at level k from a pre-exchange field (layout
+// lc_index) with element stride es (own value, then the sources in the
+// reference's order); gather_lf_load issues the four loads, gather_sum adds
+// them (FP32 buffers of the mixed form are accumulated in FP64 the same way)
 template <typename T>
-__device__ __forceinline__ void gather_lf_load(const T *f, size_t es, size_t off, int L, int ie, int pt, int k, int s0,
+__device__ __forceinline__ void gather_lf_load(const T *f, size_t es, size_t off, int ie, int pt, int k, int s0,
                                                int s1, int s2, T (&r)[4]) {
-  const size_t own = (size_t)ie * es + off + (size_t)pt * L + k;
-  auto at = [&](int s) { return opaque_index(s >= 0 ? (size_t)(s >> 4) * es + off + (size_t)(s & 15) * L + k : own); };
+  const size_t own = (size_t)ie * es + off + lc_index(pt, k);
+  auto at = [&](int s) { return opaque_index(s >= 0 ? (size_t)(s >> 4) * es + off + lc_index(s & 15, k) : own); };
   r[0] = f[own];
   r[1] = f[at(s0)];
   r[2] = f[at(s1)];
@@ -120,10 +120,10 @@ __device__ __forceinline__ void gather_lf_load(const T *f, size_t es, size_t off
 }
 
 template <typename T>
-__device__ __forceinline__ double gather_lf(const T *f, size_t es, size_t off, int L, int ie, int pt, int k, int s0,
-                                            int s1, int s2) {
+__device__ __forceinline__ double gather_lf(const T *f, size_t es, size_t off, int ie, int pt, int k, int s0, int s1,
+                                            int s2) {
   T r[4];
-  gather_lf_load(f, es, off, L, ie, pt, k, s0, s1, s2, r);
+  gather_lf_load(f, es, off, ie, pt, k, s0, s1, s2, r);
   return gather_sum(r, s0, s1, s2);
 }
 
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
   const int gkc = gon ? gk : L - 1;  // gathers of a padding lane stay in bounds
   if (GATHER) {
     // assembled divdp_proj of stage 1
-    const double y = gather_lf(a.fdss_pre_in, (size_t)16 * L, 0, L, ie, gpt, gkc, g0, g1, g2) * gr;
+    const double y = gather_lf(a.fdss_pre_in, lc_field(L), 0, ie, gpt, gkc, g0, g1, g2) * gr;
     if (gon) tile[0][gkk][gpt] = y;
     lds_barrier();
     dpproj = tile[0][kk][pt];
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
     const double diss_fac = -a.rhs_viss * a.dt * a.nu_q;
     dpdissk += diss_fac * a.dpdiss_bih[s] / e.sph;
   }
-  if (son) a.fdss_pre[((size_t)ie * 16 + pt) * L + k] = a.fdss[s] * e.sph;  // [ie][pt][lev]
+  if (son) a.fdss_pre[(size_t)ie * lc_field(L) + lc_index(pt, k)] = a.fdss[s] * e.sph;
   const bool add_hv = !GATHER && a.rhs_viss != 0.0;
   const bool limit = a.limiter_option == 8 || a.limiter_option == 9;
   DvvLaneF df;
@@ -266,9 +266,9 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
   }
   const double c = e.sph * dpdissk;
   const double s1 = limit ? slab_sum_serial(c) : 0.0;  // same for every tracer
-  const size_t es = (size_t)Q * 16 * L;
+  const size_t fs = lc_field(L), es = (size_t)Q * fs;
   double graw[4] = {};  // the loads of the next tracer's gather
-  if (GATHER && Q > 0) gather_lf_load(a.qdp_pre_in, es, 0, L, ie, gpt, gkc, g0, g1, g2, graw);
+  if (GATHER && Q > 0) gather_lf_load(a.qdp_pre_in, es, 0, ie, gpt, gkc, g0, g1, g2, graw);
   const int kc = son ? k : L - 1;  // reads of a padding slab stay in bounds
   // Slab path (stages 1 and 3): tracer q+1's Qdp, biharmonic term and
   // bounds are loaded during tracer q's arithmetic.  The loads are
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
       if (gon) tile[q & 1][gkk][gpt] = y;
       // (unconditional, the last tracer reloads itself: a branch here would
       // make every later wait cover the gathers)
-      gather_lf_load(a.qdp_pre_in, es, (size_t)(q + 1 < Q ? q + 1 : q) * 16 * L, L, ie, gpt, gkc, g0, g1, g2, graw);
+      gather_lf_load(a.qdp_pre_in, es, (size_t)(q + 1 < Q ? q + 1 : q) * fs, ie, gpt, gkc, g0, g1, g2, graw);
       lds_barrier();
       qd = tile[q & 1][kk][pt];
     } else if (!PF) {
@@ -367,14 +367,15 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
       if (wmin) ql[k] = minp;
       if (wmax) ql[L + k] = maxp;
     }
-    if (son) a.qdp_pre[(((size_t)ie * Q + q) * 16 + pt) * L + k] = e.sph * ptens;  // [ie][q][pt][lev]
+    if (son) a.qdp_pre[((size_t)ie * Q + q) * fs + lc_index(pt, k)] = e.sph * ptens;
   }
 }
 
 // ---------------------------------------------------------------- exchanges
 // The pre-exchange buffers (qdp_pre, qbih_pre, fdss_pre) are stored levels
-// fastest, [ie][q][pt][lev], so a neighbour copy of 16 consecutive levels of
-// one GLL point is one contiguous 128-byte run.  An exchange block is one
+// fastest in chunks of 8 levels (lc_index), so a neighbour copy of 16
+// consecutive levels of one GLL point is two 64-byte runs, each half of a
+// line whose other half the same block reads.  An exchange block is one
 // element x 16 levels (XL): lane (pt, kk) gathers point pt at level k0+kk,
 // then the assembled values go through an LDS tile to lanes (kk, pt) and are
 // stored in the slab layout [ie][q][lev][pt].  Blocks walk the elements of
@@ -419,7 +420,7 @@ __global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int avg) {
   const int s0 = src[0], s1 = src[1], s2 = src[2];
   const double r = a.m.rspheremp[(size_t)x.ie * 16 + x.pt];
   const bool gk = x.k < L, wkv = x.k0 + x.wk < L;
-  const size_t es = (size_t)Q * 16 * L;
+  const size_t fs = lc_field(L), es = (size_t)Q * fs;
   const int nq = (Q - q0) < QB ? (Q - q0) : QB;
   const int kc = gk ? x.k : L - 1;  // gathers of a padding lane stay in bounds
   // every load of the block's gathers is issued before the first sum
@@ -427,9 +428,9 @@ __global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int avg) {
   if (nq > 0) {
 #pragma unroll
     for (int i = 0; i < QB; ++i)
-      v[i] = gather_lf(a.qdp_pre, es, (size_t)(q0 + (i < nq ? i : 0)) * 16 * L, L, x.ie, x.pt, kc, s0, s1, s2);
+      v[i] = gather_lf(a.qdp_pre, es, (size_t)(q0 + (i < nq ? i : 0)) * fs, x.ie, x.pt, kc, s0, s1, s2);
   }
-  if (q0 == 0) gather_lf_load(a.fdss_pre, (size_t)16 * L, 0, L, x.ie, x.pt, kc, s0, s1, s2, fr);
+  if (q0 == 0) gather_lf_load(a.fdss_pre, fs, 0, x.ie, x.pt, kc, s0, s1, s2, fr);
 #pragma unroll
   for (int i = 0; i < QB; ++i) {
     if (i >= nq) break;
@@ -490,7 +491,7 @@ __global__ __launch_bounds__(256) void euler_qmm_lap_kernel(EulerArgs a) {
       ql[L + k] = mx;
     }
     const double y = scale ? x * wdp / dp0 : x;
-    const size_t o = (((size_t)ie * Q + q) * 16 + pt) * L + k;  // [ie][q][pt][lev]
+    const size_t o = ((size_t)ie * Q + q) * lc_field(L) + lc_index(pt, k);
     if (MIXED) ((float *)a.qbih_pre)[o] = sph_laplacef(ef, df, (float)y);
     else a.qbih_pre[o] = sph_laplace(e, d, y);
   };
@@ -515,16 +516,16 @@ __global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
   const bool gk = x.k < L;
   const int wk = x.k0 + x.wk;
   const bool wkv = wk < L;
-  const size_t es = (size_t)Q * 16 * L;
+  const size_t fs = lc_field(L), es = (size_t)Q * fs;
   const int nq = (Q - q0) < QB ? (Q - q0) : QB;
   const int kc = gk ? x.k : L - 1;  // gathers of a padding lane stay in bounds
   // every load of the block's gathers is issued before the first sum
   double v[QB];
 #pragma unroll
   for (int i = 0; i < QB; ++i) {
-    const size_t off = (size_t)(q0 + (i < nq ? i : 0)) * 16 * L;
-    if (MIXED) v[i] = gather_lf((const float *)a.qbih_pre, es, off, L, x.ie, x.pt, kc, s0, s1, s2);
-    else v[i] = gather_lf(a.qbih_pre, es, off, L, x.ie, x.pt, kc, s0, s1, s2);
+    const size_t off = (size_t)(q0 + (i < nq ? i : 0)) * fs;
+    if (MIXED) v[i] = gather_lf((const float *)a.qbih_pre, es, off, x.ie, x.pt, kc, s0, s1, s2);
+    else v[i] = gather_lf(a.qbih_pre, es, off, x.ie, x.pt, kc, s0, s1, s2);
   }
   ElemLane e;
   load_elem_light(a.m, x.ie, x.wpt, e);

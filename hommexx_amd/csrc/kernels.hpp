@@ -54,6 +54,14 @@ struct HvArgs {
   int consthv;                      // 0: tensor form of the second laplacian
 };
 
+// Layout of the tracer pre-exchange buffers (qdp_pre, qbih_pre, fdss_pre):
+// levels fastest in chunks of 8 levels, [ie][field][lev/8][pt][lev%8].  One
+// chunk of an element and field is 16 points x 8 levels (1 KB of FP64, 8
+// lines of 128 B holding two GLL points each), so the block that gathers an
+// element's 8-level group uses every line it fetches in full.
+__host__ __device__ inline size_t lc_field(int nlev) { return (size_t)((nlev + 7) >> 3) * 128; }
+__host__ __device__ inline size_t lc_index(int pt, int k) { return (size_t)(k >> 3) * 128 + pt * 8 + (k & 7); }
+
 struct EulerArgs {
   MetricPtrs m;
   const int *src, *nbr;
@@ -184,6 +192,7 @@ void launch_diag_scalars(hipStream_t st, const DiagArgs &a);
 // level and column) and a grid chunk (20 KB) fit in 64 KB of LDS
 constexpr int REMAP_MAX_NLEV = 256;
 void launch_remap(hipStream_t st, const RemapArgs &a);
+void launch_ps_from_dp(hipStream_t st, int nelem, int nlev, double hyai0, double ps0, const double *dp, double *ps);
 void launch_zero_int(hipStream_t st, int *p);
 
 // Streaming (non-temporal) store for outputs that only a later kernel reads:

@@ -378,6 +378,22 @@ __global__ void zero_int_kernel(int *p) { *p = 0; }
 // a fill kernel rather than a memset node, so the step stays graph-capturable
 void launch_zero_int(hipStream_t st, int *p) { hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, st, p); }
 
+// ps_v = sum(dp3d) + hyai0*ps0, summed as remap_thickness_kernel does
+__global__ __launch_bounds__(256) void ps_from_dp_kernel(int nelem, int nlev, double hyai0, double ps0,
+                                                          const double *dp, double *ps) {
+  const size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= (size_t)nelem * 16) return;
+  const double *d = dp + (col >> 4) * nlev * 16 + (col & 15);
+  double s = 0.0;
+  for (int k = 0; k < nlev; ++k) s += d[(size_t)k * 16];
+  ps[col] = s + hyai0 * ps0;
+}
+
+void launch_ps_from_dp(hipStream_t st, int nelem, int nlev, double hyai0, double ps0, const double *dp, double *ps) {
+  const size_t ncol = (size_t)nelem * 16;
+  hipLaunchKernelGGL(ps_from_dp_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, nelem, nlev, hyai0, ps0, dp, ps);
+}
+
 void launch_remap(hipStream_t st, const RemapArgs &a) {
   const size_t ncol = (size_t)a.nelem * 16;
   hipLaunchKernelGGL(remap_thickness_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, a);

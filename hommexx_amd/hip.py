@@ -81,7 +81,10 @@ class HipSession:
 
     FORMS = {"fp64": 0, "mixed": 1}
 
-    def __init__(self, prob, abort_on_error=False, form="fp64"):
+    def __init__(self, prob, abort_on_error=False, form="fp64", configure=True):
+        """``configure=False`` leaves nlev and qsize_d at the library's build
+        defaults (PREQX_PLEV, QSIZE_D), as an unchanged Fortran driver does;
+        ``prob`` must then match them."""
         L = load()
         if HipSession._alive:
             raise HommexxError("a HipSession is already active; close() it first")
@@ -95,9 +98,10 @@ class HipSession:
         self._check("initialize_hommexx_session")
         HipSession._alive = True
         self._closed = False
-        rc = L.hommexx_hip_configure(prob.nlev, prob.qsize_d)
-        if rc:
-            raise HommexxError("hommexx_hip_configure failed (%d)" % rc)
+        if configure:
+            rc = L.hommexx_hip_configure(prob.nlev, prob.qsize_d)
+            if rc:
+                raise HommexxError("hommexx_hip_configure failed (%d)" % rc)
         p = prob.params
         # prim_init1: connectivity
         L.reset_cxx_comm(_r(0, _I))

@@ -18,6 +18,7 @@ Fortran shapes, e.g. ``v`` is ``(nelem, 3, nlev, 2, 4, 4)`` for Fortran
 import json
 import math
 import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -69,6 +70,19 @@ class VCoord:
         return da * self.ps0 + db * ps
 
 
+def _by_elements(fn, nelem):
+    """fn(lo, hi) over element ranges on a few host threads (numpy releases
+    the GIL in its array loops); for the multi-GB tracer arrays of large
+    meshes"""
+    nt = max(1, min(16, os.cpu_count() or 1, nelem // 256))
+    if nt == 1:
+        fn(0, nelem)
+        return
+    cuts = [nelem * i // nt for i in range(nt + 1)]
+    with ThreadPoolExecutor(nt) as ex:
+        list(ex.map(lambda i: fn(cuts[i], cuts[i + 1]), range(nt)))
+
+
 def _smooth_tracers(mesh, nlev, qsize, seed, out=None):
     """Seeded smooth tracers in [0, 1]: a random l<=2 harmonic pattern per
     tracer, scaled into [0.05, 0.95], times a gentle vertical modulation."""
@@ -87,7 +101,11 @@ def _smooth_tracers(mesh, nlev, qsize, seed, out=None):
         phase = rng.uniform(0, 2 * math.pi)
         vert = 0.8 + 0.2 * np.cos(2 * math.pi * kk / max(nlev, 1) + phase)  # in [0.6, 1]
         base = 0.5 + 0.45 * h  # in [0.05, 0.95]
-        out[:, q] = base[:, None, :, :] * vert[None, :, None, None]
+
+        def fill(lo, hi, q=q, base=base, vert=vert):
+            np.multiply(base[lo:hi, None, :, :], vert[None, :, None, None], out=out[lo:hi, q])
+
+        _by_elements(fill, mesh.nelem)
     return out
 
 
@@ -138,8 +156,12 @@ def jw_baroclinic(mesh, vc, qsize, seed=0, qsize_d=None, t_noise=0.0, u_perturb=
     if qsize > 0:
         _smooth_tracers(mesh, nlev, qsize, seed, out=Q[:, :qsize])
     Qdp = np.empty((nelem, 2, qsize_d, nlev, 4, 4))
-    np.multiply(Q, dp[:, None], out=Qdp[:, 0])
-    Qdp[:, 1] = Qdp[:, 0]
+
+    def qdp(lo, hi):
+        np.multiply(Q[lo:hi], dp[lo:hi, None], out=Qdp[lo:hi, 0])
+        Qdp[lo:hi, 1] = Qdp[lo:hi, 0]
+
+    _by_elements(qdp, nelem)
     return {
         "v": np.ascontiguousarray(v), "T": np.ascontiguousarray(T),
         "dp3d": np.ascontiguousarray(dp3d), "ps_v": ps_v, "phis": phis,

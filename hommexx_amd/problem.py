@@ -72,20 +72,23 @@ class Problem:
 _MESH_CACHE = {}
 
 
-def build_problem(ne=4, nlev=26, qsize=4, seed=0, dt=300.0, t_noise=0.0, u_perturb=1.0, **param_overrides):
+def build_problem(ne=4, nlev=26, qsize=4, seed=0, dt=300.0, t_noise=0.0, u_perturb=1.0, qsize_d=None,
+                  **param_overrides):
     """Synthetic JW-baroclinic problem on an ``ne`` cubed sphere.
 
     Defaults follow the small CPU configuration (ne=4, L26, Q4, dt=300 s,
     nu=1e16, hypervis subcycle 3, remap every step, limiter 8, remap alg 1,
     no forcing).  ``u_perturb`` scales the 1 m/s Gaussian jet perturbation
-    (0 gives the steady balanced jet).
+    (0 gives the steady balanced jet).  ``qsize_d`` is the tracer dimension
+    of the Fortran arrays (QSIZE_D, default ``max(qsize, 1)``); slots beyond
+    ``qsize`` hold seeded data that the step must carry unchanged.
     """
     if ne not in _MESH_CACHE:
         _MESH_CACHE[ne] = CubedSphere(ne)
     mesh = _MESH_CACHE[ne]
     vc = VCoord(nlev)
     p = Params(qsize=qsize, **param_overrides)
-    state = jw_baroclinic(mesh, vc, qsize, seed=seed, t_noise=t_noise, u_perturb=u_perturb)
+    state = jw_baroclinic(mesh, vc, qsize, seed=seed, qsize_d=qsize_d, t_noise=t_noise, u_perturb=u_perturb)
     return Problem(ne=ne, nlev=nlev, qsize=qsize, dt=dt, params=p, mesh=mesh, vcoord=vc,
                    dvv=np.ascontiguousarray(derivative_matrix()), metric=mesh.fortran_arrays(),
                    state=state, connections=list(mesh.connections))

@@ -109,9 +109,14 @@ void f90_push_forcing_to_cxx(double *FM, double *FT, double *FQ, double *Qdp);
  * ------------------------------------------------------------------------- */
 
 /* nlev and qsize_d are compile-time constants in the reference (PLEV,
- * QSIZE_D); set them before init_elements_2d_c.  Defaults 72 and qsize.
+ * QSIZE_D; cmake -DPREQX_PLEV, -DQSIZE_D, defaults 72 and 4).  The library
+ * takes its defaults from the same build variables (make PREQX_PLEV=...
+ * QSIZE_D=...), so an unchanged Fortran driver needs no call here; a harness
+ * may override them before init_elements_2d_c (qsize_d <= 0: qsize).
  * Returns 0, 1 if elements already exist, 2 if nlev is outside 3..256. */
 int hommexx_hip_configure(int nlev, int qsize_d);
+/* the nlev and qsize_d the next init_elements_2d_c will use */
+void hommexx_hip_dimensions(int *nlev, int *qsize_d);
 /* 1 (default): abort like the reference; 0: record the error and return */
 void hommexx_hip_set_abort_on_error(int on);
 /* 0 (default): the FP64 step.  1: the mixed-precision form -- the sphere
@@ -148,9 +153,21 @@ void hommexx_hip_time_levels(int *nm1, int *n0, int *np1, int *nstep, int *n0_qd
  * and the FP64 form, and leaves the session's model state unchanged.
  * Return 0 on success. */
 int hommexx_hip_get_state(double *v, double *T, double *ps);
+/* one field of the current state into device memory: which 0 = v, 1 = T,
+ * 2 = dp3d, 3 = ps_v (time level n0), 4 = Qdp of tracer q (the tracer level
+ * the next step starts from), 5 = Q of tracer q; layouts as above, tracer
+ * fields [ie][lev][pt].  For comparisons on the device (no host copies of
+ * large states).  Returns 0 on success. */
+int hommexx_hip_get_field(int which, int q, double *dst);
 int hommexx_hip_set_state(const double *v, const double *T, const double *ps);
 int hommexx_hip_adjoint_step(double dt, const double *v, const double *T, const double *ps, double *lam_v,
                              double *lam_T, double *lam_ps);
+/* the map hommexx_hip_adjoint_step differentiates, forward: (v, T, ps) is
+ * advanced in place by one prim_run_subcycle_c step of length dt without its
+ * tracer step (for a dry run the (v, T, ps) that step produces); used to
+ * replay checkpointed segments.  Same conditions; the session's model state
+ * is not changed.  Returns 0 on success. */
+int hommexx_hip_dynamics_step(double dt, double *v, double *T, double *ps);
 
 #ifdef __cplusplus
 }

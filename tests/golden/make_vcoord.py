@@ -4,9 +4,10 @@ The numbers are data read from the reference's ASCII coordinate files
 (test/vcoord/{acme-72i,acme-72m,cami-26,camm-26}.ascii): each file holds a
 count line followed by that many coefficients (interfaces: hyai then hybi;
 midpoints: hyam then hybm).  This script is run by hand in a container that
-has the reference mounted; the JSON it writes is what the package loads.
+has the reference mounted; the JSON it writes into hommexx_amd/data/ is what
+the package loads.
 
-    python hommexx_amd/data/make_vcoord.py [/path/to/reference]
+    python tests/golden/make_vcoord.py [/path/to/reference]
 """
 import hashlib
 import json
@@ -44,7 +45,8 @@ def _is_float(t):
 
 
 def main(ref_root):
-    here = os.path.dirname(os.path.abspath(__file__))
+    out_dir = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))),
+                        "hommexx_amd", "data")
     for nlev, (fi, fm) in SOURCES.items():
         pi = os.path.join(ref_root, fi)
         pm = os.path.join(ref_root, fm)
@@ -58,7 +60,7 @@ def main(ref_root):
             "source": [fi, fm],
             "sha256": [hashlib.sha256(open(p, "rb").read()).hexdigest() for p in (pi, pm)],
         }
-        with open(os.path.join(here, "vcoord_L%d.json" % nlev), "w") as f:
+        with open(os.path.join(out_dir, "vcoord_L%d.json" % nlev), "w") as f:
             json.dump(rec, f, indent=1)
         print("wrote vcoord_L%d.json" % nlev)
 

@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -61,3 +62,28 @@ def test_configure_bounds_nlev():
             "print([L.hommexx_hip_configure(n, 4) for n in (2, 3, 72, 256, 257, 4096)])")
     out = subprocess.check_output([os.sys.executable, "-c", code, LIB]).decode().strip()
     assert out == "[2, 0, 0, 0, 2, 2]", out
+
+
+def _dims(lib):
+    code = ("import ctypes, sys; L = ctypes.CDLL(sys.argv[1]); n = ctypes.c_int(); q = ctypes.c_int(); "
+            "L.hommexx_hip_dimensions(ctypes.byref(n), ctypes.byref(q)); print(n.value, q.value)")
+    out = subprocess.check_output([sys.executable, "-c", code, lib]).decode().split()
+    return int(out[0]), int(out[1])
+
+
+def test_build_dimensions_default_like_the_reference():
+    """Without hommexx_hip_configure the library uses the build's PREQX_PLEV
+    and QSIZE_D, whose defaults are the reference's (72 and 4,
+    src/preqx/CMakeLists.txt:244-261)."""
+    if not os.path.exists(LIB):
+        pytest.skip("libhommexx_amd.so not built")
+    assert _dims(LIB) == (72, 4)
+
+
+def test_build_dimensions_follow_make_variables(tmp_path):
+    """make PREQX_PLEV=26 QSIZE_D=6 (as cmake -DPREQX_PLEV=26 -DQSIZE_D=6
+    builds the reference) gives a library whose defaults are 26 and 6."""
+    lib = str(tmp_path / "libhx_L26_Q6.so")
+    subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "hommexx_amd", "csrc"), "-j8",
+                           "BUILD=%s" % tmp_path, "LIB=%s" % lib, "PREQX_PLEV=26", "QSIZE_D=6"])
+    assert _dims(lib) == (26, 6)

@@ -1,0 +1,96 @@
+"""The large single-GPU configurations, compared on the device (no host
+copies of the state):
+
+* ne60 L72 Q40 (21,600 elements), the mixed-precision form against the FP64
+  form over 200 steps, with 0.1 K seeded white noise in T: every prognostic
+  field within 1e-6 relative L2, and the global dry and tracer mass drifts.
+  dt is 150 s (ne30's 300 s halved for the advective limit) and nu 1.1e14
+  (ne30's 1e15 scaled by the grid spacing to the power 3.2, as the
+  reference's resolution-dependent settings do: xx-ne30-v1.nl:19-20,35).
+* ne120 L72 Q40 (86,400 elements, the largest case on one GPU), both forms
+  for a few steps at dt 75 s (the reference's ne120 step) and nu 1.2e13.
+
+Each configuration runs the FP64 form first, keeps its fields on the device,
+then runs the mixed form from the same initial state in a new session.  The
+FP64 form itself is pinned to the oracle at ne30 (test_gpu_parity_ne30.py).
+"""
+import time
+
+import pytest
+import torch
+
+from device_fields import masses, rel_errors, snapshot
+from hommexx_amd.hip import HipSession
+from hommexx_amd.problem import build_problem
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # ne, dt, nu, steps
+    "ne60": (60, 150.0, 1.1e14, 200),
+    "ne120": (120, 75.0, 1.2e13, 10),
+}
+
+
+def make(name):
+    ne, dt, nu, steps = CASES[name]
+    prob = build_problem(ne=ne, nlev=72, qsize=40, seed=3, dt=dt, nu=nu, nu_p=nu, nu_q=nu, nu_s=nu, nu_div=nu,
+                         nu_top=2.5e5, hypervis_subcycle=3, rsplit=1, t_noise=0.1)
+    return prob, steps
+
+
+@pytest.fixture(scope="module")
+def runs():
+    return {}
+
+
+def drift(m0, m1):
+    return abs(m1[0] / m0[0] - 1.0), float(torch.max(torch.abs(m1[1] / m0[1] - 1.0)))
+
+
+@pytest.mark.parametrize("name", ["ne60", "ne120"])
+def test_large_fp64(runs, name):
+    prob, steps = make(name)
+    tracers = ("Qdp", "Q") if name == "ne60" else ("Qdp",)
+    with HipSession(prob, form="fp64") as h:
+        m0 = masses(h)
+        h.run(1)
+        h.synchronize()
+        t0 = time.perf_counter()
+        h.run(steps - 1)
+        h.synchronize()
+        ms = (time.perf_counter() - t0) / (steps - 1) * 1e3
+        m1 = masses(h)
+        ref = snapshot(h, tracers)
+    torch.cuda.empty_cache()
+    dd, dq = drift(m0, m1)
+    print("%s L72 Q40 FP64, %d steps: %.2f ms/step; dry mass drift %.2e, tracer mass drift %.2e"
+          % (name, steps, ms, dd, dq))
+    assert dd <= 1e-13 and dq <= 1e-12
+    runs[name] = (prob, steps, ref, ms)
+
+
+@pytest.mark.parametrize("name", ["ne60", "ne120"])
+def test_large_mixed(runs, name):
+    if name not in runs:
+        pytest.skip("the FP64 run of this case did not complete")
+    prob, steps, ref, ms64 = runs.pop(name)
+    torch.cuda.empty_cache()  # the library allocates with hipMalloc, outside torch's cache
+    with HipSession(prob, form="mixed") as h:
+        m0 = masses(h)
+        h.run(1)
+        h.synchronize()
+        t0 = time.perf_counter()
+        h.run(steps - 1)
+        h.synchronize()
+        ms = (time.perf_counter() - t0) / (steps - 1) * 1e3
+        m1 = masses(h)
+        err = rel_errors(h, ref)
+    del ref
+    dd, dq = drift(m0, m1)
+    print("%s L72 Q40 mixed, %d steps: %.2f ms/step (%.3fx the FP64 form); dry mass drift %.2e, tracer mass "
+          "drift %.2e; rel L2 vs FP64: %s" % (name, steps, ms, ms64 / ms, dd, dq,
+                                               ", ".join("%s %.2e" % kv for kv in err.items())))
+    for k, e in err.items():
+        assert e <= 1e-6, (name, k, e)
+    assert dd <= 1e-13 and dq <= 1e-11

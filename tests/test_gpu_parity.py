@@ -170,3 +170,24 @@ def test_mass_conservation_ne8():
     q = np.einsum("epq,eclpq->c", sph, st["Qdp"][:, nstep % 2])[:2]
     assert abs(dry / dry0 - 1) < 1e-14
     assert np.all(np.abs(q / q0 - 1) < 1e-14)
+
+
+@pytest.mark.parametrize("qsize", [9, 17, 40, 62])
+def test_tracer_batch_parity(qsize):
+    """Tracer counts that reach every batching layout of the tracer kernels:
+    9 and 17 leave a partial second and third batch of eight in the
+    exchange kernels (euler_dss, bih_post); the remap sweeps qsize + 3
+    fields at 1, 2, 3 and 4 fields per lane (12, 20, 43, 65 fields), 65
+    needing a second block per element."""
+    prob = build_problem(ne=4, nlev=26, qsize=qsize, seed=20 + qsize)
+    print("case: ne4 L26 Q%d" % qsize, flush=True)
+    o = oracle.OracleSession(prob)
+    with HipSession(prob) as h:
+        for n, tol in ((1, TOL1), (9, TOL10)):
+            o.run(n)
+            h.run(n)
+            rep = compare(h.state(), o.state())
+            print(qsize, 1 if n == 1 else 10, "steps:", fmt(rep))
+            for k, (e, _) in rep.items():
+                assert e <= tol, (qsize, n, k, e)
+    o.close()

@@ -60,16 +60,40 @@ def test_adjoint_matches_torch_reference_ne4(rsplit, alg, qsplit):
 
 
 @pytest.mark.parametrize("rsplit", [0, 1, 3])
+def test_dynamics_step_is_the_forward_step(rsplit):
+    """hommexx_hip_dynamics_step (the backward pass's replay, no tracer step)
+    gives the (v, T, ps) of prim_run_subcycle_c bit for bit in a dry run with
+    tracers, and leaves the session's state alone."""
+    prob = build_problem(ne=4, nlev=26, qsize=3, seed=34, rsplit=rsplit, t_noise=0.1)
+    with DiffSession(prob) as s:
+        for n in range(3):
+            x = s.get_state()
+            y = tuple(t.clone() for t in x)
+            s.dynamics_step(y)
+            assert all(torch.equal(a, b) for a, b in zip(s.get_state(), x))
+            s.step()
+            z = s.get_state()
+            for a, b in zip(y, z):
+                assert torch.equal(a, b), (rsplit, n)
+
+
+@pytest.mark.parametrize("rsplit", [0, 1, 3])
 def test_adjoint_checkpoint_interval_independent(rsplit):
-    """The gradient does not depend on the checkpoint interval."""
-    prob = build_problem(ne=4, nlev=26, qsize=0, seed=33, rsplit=rsplit)
+    """The gradient does not depend on the checkpoint interval (every = 1
+    takes each step's input from the forward pass's checkpoints; 3 and 5
+    replay the others with the dynamics-only step, tracers present), and the
+    backward pass leaves the session at the forward pass's final state."""
+    prob = build_problem(ne=4, nlev=26, qsize=3, seed=33, rsplit=rsplit)
     tgt, w = synthetic_target(prob, 2), region_weight(prob)
     grads = []
     with DiffSession(prob) as s:
         for every in (1, 3, 5):
             x = [a.clone().requires_grad_() for a in initial_state(prob)]
-            J = energy_mismatch(prob, *trajectory(s, *x, 5, every), tgt, w)
+            out = trajectory(s, *x, 5, every)
+            J = energy_mismatch(prob, *out, tgt, w)
             grads.append(torch.autograd.grad(J, x))
+            for a, b in zip(s.get_state(), out):
+                assert torch.equal(a, b)
     for g in grads[1:]:
         for a, b in zip(g, grads[0]):
             assert torch.equal(a, b)
