@@ -188,7 +188,17 @@ def run_adjoint(rsplit, steps=10, traj_steps=48, every=8):
             s.adjoint_step(st, lam)
         torch.cuda.synchronize()
         adj = (time.perf_counter() - t0) / steps
-        progress("adjoint: forward %.3f ms/step, adjoint %.3f ms/step" % (fwd * 1e3, adj * 1e3))
+        # the dry dynamics alone (no tracer step): the map the adjoint differentiates
+        y = tuple(a.clone() for a in st)
+        s.dynamics_step(y)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.dynamics_step(y)
+        torch.cuda.synchronize()
+        dyn = (time.perf_counter() - t0) / steps
+        progress("adjoint: forward %.3f ms/step (dynamics only %.3f), adjoint %.3f ms/step"
+                 % (fwd * 1e3, dyn * 1e3, adj * 1e3))
         tgt, w = synthetic_target(prob, 6), region_weight(prob)
         # warm-up: a 2-step trajectory and its gradient (first-use costs of
         # the loss's torch kernels and of the checkpoint allocations)
@@ -209,6 +219,7 @@ def run_adjoint(rsplit, steps=10, traj_steps=48, every=8):
                        "nelem": prob.nelem},
             "forward_step_ms": round(fwd * 1e3, 3), "adjoint_step_ms": round(adj * 1e3, 3),
             "adjoint_over_forward": round(adj / fwd, 3),
+            "dynamics_step_ms": round(dyn * 1e3, 3), "adjoint_over_dynamics": round(adj / dyn, 3),
             "trajectory": {"steps": traj_steps, "checkpoint_every": every,
                            "forward_s": round(t1 - t0, 4), "backward_s": round(t2 - t1, 4),
                            "backward_over_forward": round((t2 - t1) / (t1 - t0), 3)}}

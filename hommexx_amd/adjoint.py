@@ -3,14 +3,15 @@ the C-ABI.
 
 The forward pass runs ``prim_run_subcycle_c`` (the HIP step) and keeps the
 prognostic state (v, T, ps) every ``every`` steps as device checkpoints.  The
-backward pass replays each segment from its checkpoint with
-``hommexx_hip_dynamics_step`` (the same map without the tracer step, which a
-dry run's dynamics do not see), keeping the input state of every step of the
-segment, and runs ``hommexx_hip_adjoint_step`` (hand-written HIP adjoint
-kernels of CAAR, hyperviscosity, DSS and the PPM remap) over the segment's
-steps in reverse.  Neither call touches the session's model state, which
-stays at the forward pass's end.  Device memory: ``ceil(nsteps/every)``
-checkpoints plus ``every`` step states, each (v, T, ps).
+backward pass hands each segment, last first, to
+``hommexx_hip_adjoint_segment``: from the segment's checkpoint it recomputes
+the dynamics of the segment's steps once (the same map without the tracer
+step, which a dry run's dynamics do not see), keeping every stage state, and
+runs the hand-written HIP adjoint kernels of CAAR, hyperviscosity, DSS and
+the PPM remap over the steps in reverse.  The session's model state is not
+touched and stays at the forward pass's end.  Device memory:
+``ceil(nsteps/every)`` checkpoints (v, T, ps) plus the stage states of one
+segment inside the library.
 
 The adjoint covers dry configurations with any ``rsplit`` and ``qsplit``: one
 ``prim_run_subcycle_c`` call is qsplit * max(rsplit, 1) dynamics steps (RK5
@@ -57,6 +58,7 @@ class DiffSession:
         L.hommexx_hip_set_state.argtypes = [_P, _P, _P]
         L.hommexx_hip_adjoint_step.argtypes = [_D, _P, _P, _P, _P, _P, _P]
         L.hommexx_hip_dynamics_step.argtypes = [_D, _P, _P, _P]
+        L.hommexx_hip_adjoint_segment.argtypes = [_D, ctypes.c_int, _P, _P, _P, _P, _P, _P]
         self.L = L
         E, nl = prob.nelem, prob.nlev
         self.shapes = ((E, nl, 2, 4, 4), (E, nl, 4, 4), (E, 4, 4))
@@ -97,6 +99,16 @@ class DiffSession:
             self.h._check("hommexx_hip_adjoint_step")
             raise HommexxError("hommexx_hip_adjoint_step failed (%d)" % rc)
 
+    def adjoint_segment(self, state, n, lam):
+        """lam (adjoint of the state after n steps from ``state``) -> adjoint
+        of ``state``, in place"""
+        torch.cuda.synchronize(self.device)
+        rc = self.L.hommexx_hip_adjoint_segment(self.prob.dt, n, *[_ptr(t) for t in state],
+                                                *[_ptr(t) for t in lam])
+        if rc:
+            self.h._check("hommexx_hip_adjoint_segment")
+            raise HommexxError("hommexx_hip_adjoint_segment failed (%d)" % rc)
+
     def close(self):
         self.h.close()
 
@@ -130,13 +142,7 @@ class PreqxTrajectory(torch.autograd.Function):
         for s in reversed(range(len(ctx.ckpts))):
             n0 = s * every
             n1 = min(n0 + every, nsteps)
-            states = [ctx.ckpts[s]]
-            for _ in range(n0, n1 - 1):
-                nxt = tuple(t.clone() for t in states[-1])
-                sess.dynamics_step(nxt)
-                states.append(nxt)
-            for st in reversed(states):
-                sess.adjoint_step(st, lam)
+            sess.adjoint_segment(ctx.ckpts[s], n1 - n0, lam)
         ctx.ckpts = None
         return lam[0], lam[1], lam[2], None, None, None
 

@@ -326,8 +326,8 @@ void alloc_work() {
   c.qbih = c.qdp_pre;
   c.qbih_pre = dalloc<double>(q3c);
   c.qdp_pre2 = c.qbih_pre;
-  c.qlim[0] = dalloc<double>((size_t)c.nelem * (c.qsize ? c.qsize : 1) * 2 * c.nlev);
-  c.qlim[1] = dalloc<double>((size_t)c.nelem * (c.qsize ? c.qsize : 1) * 2 * c.nlev);
+  c.qlim[0] = dalloc<double>((size_t)c.nelem * (c.qsize ? c.qsize : 1) * ql_row(c.nlev));
+  c.qlim[1] = dalloc<double>((size_t)c.nelem * (c.qsize ? c.qsize : 1) * ql_row(c.nlev));
   c.fdss_pre = dalloc<double>(e3c);
   c.fdss_pre2 = c.ct;
   c.err_d = dalloc<int>(1);
@@ -631,22 +631,43 @@ void remap(double dt_remap) {
 // Reverse-mode derivative of one dry rsplit = 0 step (no tracers on the
 // dynamics path).  The forward stage states are recomputed into private
 // buffers first; the step's model state is not touched.
+// The stage states of one dynamics step: CAAR stage inputs U0..U4, the
+// combination w = (5 u1 - u0)/4 and the HV cycle states H0..Hnsub (U0 is the
+// step's input, H[nsub] its output)
+struct Stages {
+  std::vector<StateBuf> U, H;
+  StateBuf W{};
+};
+
 struct AdjBufs {
   size_t f3 = 0;  // doubles in one 3-D scalar
   int nsub = 0;
   int nd = 0;  // dynamics steps per prim_run_subcycle_c
-  std::vector<StateBuf> U, H;  // CAAR stage inputs U0..U4, HV cycle states H0..Hnsub
+  Stages cur;                  // the stage states of one dynamics step
   std::vector<StateBuf> X;     // input states of the nd dynamics steps
-  StateBuf W{}, LX{}, L0{}, L1{}, LP{}, S{}, M{};
+  StateBuf LX{}, L0{}, L1{}, LP{}, S{}, M{};
   double *div, *p, *cc, *stot, *pb, *phib, *etab, *ombp, *divb, *lv, *lt, *ldp;
   double *rscr = nullptr;  // remap adjoint scratch (rsplit > 0)
   double *base = nullptr;
 };
 AdjBufs *adjb = nullptr;
 
+// hommexx_hip_adjoint_segment: the stage states of every dynamics step of a
+// segment of calls, so the reverse sweep recomputes none of them
+struct SegBufs {
+  size_t f3 = 0;
+  int nsub = 0, nsets = 0;
+  std::vector<Stages> sets;
+  StateBuf x{};  // the input of the next call (forward pass)
+  double *ps = nullptr;
+};
+SegBufs *segb = nullptr;
+
 void adj_reset() {
   delete adjb;
   adjb = nullptr;
+  delete segb;
+  segb = nullptr;
 }
 
 StateBuf carve_state(double *&q, size_t f3) {
@@ -673,13 +694,13 @@ void adj_layout(AdjBufs &b, double *base, int nd) {
   b.nsub = nsub;
   b.nd = nd;
   double *q = b.base;
-  b.U.clear();
-  b.H.clear();
+  b.cur.U.clear();
+  b.cur.H.clear();
   b.X.clear();
   for (int i = 0; i < nd; ++i) b.X.push_back(carve_state(q, f3));
-  for (int i = 0; i < 5; ++i) b.U.push_back(carve_state(q, f3));
-  b.W = carve_state(q, f3);
-  for (int i = 0; i <= nsub; ++i) b.H.push_back(carve_state(q, f3));
+  for (int i = 0; i < 5; ++i) b.cur.U.push_back(carve_state(q, f3));
+  b.cur.W = carve_state(q, f3);
+  for (int i = 0; i <= nsub; ++i) b.cur.H.push_back(carve_state(q, f3));
   b.LX = carve_state(q, f3);
   b.L0 = carve_state(q, f3);
   b.L1 = carve_state(q, f3);
@@ -732,10 +753,10 @@ void dss_state(const StateBuf &x, const StateBuf &y) {
 }
 
 // adjoint of out = DSS_r(sph*(a + h*RHS(b))), lam = adjoint of out
-void caar_adjoint(const StateBuf &xb, const StateBuf &lam, double h, const StateBuf &lb, bool b_acc,
+// (b: the step's buffers, scratch included)
+void caar_adjoint(AdjBufs &b, const StateBuf &xb, const StateBuf &lam, double h, const StateBuf &lb, bool b_acc,
                   const StateBuf &la1, double c1, const StateBuf *la2, double c2) {
   Ctx &c = C();
-  AdjBufs &b = *adjb;
   dss_state(lam, b.LP);
   AdjCaarArgs a{};
   a.m = metric();
@@ -781,9 +802,8 @@ void caar_adjoint(const StateBuf &xb, const StateBuf &lam, double h, const State
 }
 
 // adjoint of one hyperviscosity cycle x -> xo; lam (in place)
-void hv_adjoint(const StateBuf &x, const StateBuf &xo, const StateBuf &lam, double dt_cycle) {
+void hv_adjoint(AdjBufs &b, const StateBuf &x, const StateBuf &xo, const StateBuf &lam, double dt_cycle) {
   Ctx &c = C();
-  AdjBufs &b = *adjb;
   AdjHvArgs a{};
   a.m = metric();
   a.nelem = c.nelem;
@@ -959,27 +979,28 @@ void diag_energy(bool before_advance, int ivar) {
   hipok(hipStreamSynchronize(c.st), "prim_energy_halftimes");
 }
 
-// One dynamics step of the differentiable map from X[i]: RK5 stage inputs
-// U0..U4, w and the HV cycle states H0..Hnsub (H[nsub] is the step's output),
-// with the kernels of the forward step (no accumulators for the tracers).
-void adj_dyn_forward(AdjBufs &b, int i, double dt) {
+// One dynamics step of the differentiable map from x into the stage states
+// s, with the kernels of the forward step (no accumulators for the tracers)
+void dyn_forward_stages(const StateBuf &x, Stages &s, double dt) {
   Ctx &c = C();
-  const size_t f3 = b.f3;
+  const size_t f3 = (size_t)c.nelem * c.nlev * 16;
   const int nsub = c.p.hypervis_subcycle;
-  std::vector<StateBuf> &U = b.U, &H = b.H, &X = b.X;
-  copy_state(U[0], X[i], f3);
+  std::vector<StateBuf> &U = s.U, &H = s.H;
+  copy_state(U[0], x, f3);
   caar_run(U[0], U[0], U[1], dt / 5.0, 0.0);
   caar_run(U[0], U[1], U[2], dt / 5.0, 0.0);
   caar_run(U[0], U[2], U[3], dt / 3.0, 0.0);
   caar_run(U[0], U[3], U[4], 2.0 * dt / 3.0, 0.0);
-  copy_state(b.W, U[1], f3);
-  launch_rk_combo(c.st, c.nelem, c.nlev, b.W.v, b.W.t, b.W.dp, U[0].v, U[0].t, U[0].dp);
-  caar_run(b.W, U[4], H[0], 3.0 * dt / 4.0, 0.0);
+  copy_state(s.W, U[1], f3);
+  launch_rk_combo(c.st, c.nelem, c.nlev, s.W.v, s.W.t, s.W.dp, U[0].v, U[0].t, U[0].dp);
+  caar_run(s.W, U[4], H[0], 3.0 * dt / 4.0, 0.0);
   for (int cyc = 0; cyc < nsub; ++cyc) {
     copy_state(H[cyc + 1], H[cyc], f3);
     launch_hv_cycle(c.st, hv_args(H[cyc + 1], dt, 0.0));
   }
 }
+
+void adj_dyn_forward(AdjBufs &b, int i, double dt) { dyn_forward_stages(b.X[i], b.cur, dt); }
 
 // X[0] from (v, T, ps), then the nd dynamics steps of one
 // prim_run_subcycle_c call; H[nsub] holds the last step's output
@@ -991,7 +1012,7 @@ void adj_forward_steps(AdjBufs &b, int nd, double dt, const double *v, const dou
   launch_dp3d_from_ps(c.st, c.nelem, c.nlev, c.hyai_d, c.hybi_d, c.ps0, ps, b.X[0].dp);
   for (int i = 0; i < nd; ++i) {
     adj_dyn_forward(b, i, dt);
-    if (i + 1 < nd) copy_state(b.X[i + 1], b.H[c.p.hypervis_subcycle], f3);
+    if (i + 1 < nd) copy_state(b.X[i + 1], b.cur.H[c.p.hypervis_subcycle], f3);
   }
 }
 
@@ -1007,64 +1028,107 @@ bool adj_supported(const char *who) {
   return true;
 }
 
-// The adjoint step on the buffers b (adj_layout for nd dynamics steps)
-int adj_step_run(AdjBufs &b, int nd, double dt, const double *v, const double *T, const double *ps, double *lam_v,
-                 double *lam_T, double *lam_ps, const char *who) {
+// The reverse sweep of one prim_run_subcycle_c call: on entry lam_* hold
+// the adjoint of the call's output (v, T, ps), on return that of its input.
+// stages(i) returns the stage states of dynamics step i (recomputed or
+// stored); the last step's are asked for first.
+template <class F>
+void adj_reverse_call(AdjBufs &b, int nd, double dt, F stages, double *lam_v, double *lam_T, double *lam_ps) {
   Ctx &c = C();
   const size_t f3 = b.f3;
   const int nsub = c.p.hypervis_subcycle;
-  std::vector<StateBuf> &U = b.U, &H = b.H;
-  auto dyn_forward = [&](int i) { adj_dyn_forward(b, i, dt); };
-  // forward: the input of every dynamics step; the last step's stage states
-  // stay in U, W, H for the first reverse step
-  {
-    Scope s("adj_forward");
-    adj_forward_steps(b, nd, dt, v, T, ps);
-  }
-  // reverse sweep
-  Scope s("adj_reverse");
   hipMemcpyAsync(b.LX.v, lam_v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
   hipMemcpyAsync(b.LX.t, lam_T, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
-  if (c.p.rsplit == 0) {
-    launch_adj_fill_dp(c.st, c.nelem, c.nlev, lam_ps, b.LX.dp);  // ps = hyai(0) ps0 + sum_k dp3d
-  } else {
-    // linearised PPM remap of v and T from the thicknesses of the last step
-    AdjRemapArgs r{};
-    r.nelem = c.nelem;
-    r.nlev = c.nlev;
-    r.alg = c.p.remap_alg;
-    r.v = H[nsub].v;
-    r.t = H[nsub].t;
-    r.dp = H[nsub].dp;
-    r.l_v = b.LX.v;
-    r.l_t = b.LX.t;
-    r.l_dp = b.LX.dp;
-    r.l_ps = lam_ps;
-    r.hyai = c.hyai_d;
-    r.hybi = c.hybi_d;
-    r.ps0 = c.ps0;
-    r.scr = b.rscr;
-    launch_adj_remap(c.st, r);
-  }
   for (int i = nd - 1; i >= 0; --i) {
-    if (i < nd - 1) {
-      dyn_forward(i);  // this step's stage states
+    const Stages &st = stages(i);
+    const std::vector<StateBuf> &U = st.U, &H = st.H;
+    if (i == nd - 1) {
+      if (c.p.rsplit == 0) {
+        launch_adj_fill_dp(c.st, c.nelem, c.nlev, lam_ps, b.LX.dp);  // ps = hyai(0) ps0 + sum_k dp3d
+      } else {
+        // linearised PPM remap of v and T from the thicknesses of the last step
+        AdjRemapArgs r{};
+        r.nelem = c.nelem;
+        r.nlev = c.nlev;
+        r.alg = c.p.remap_alg;
+        r.v = H[nsub].v;
+        r.t = H[nsub].t;
+        r.dp = H[nsub].dp;
+        r.l_v = b.LX.v;
+        r.l_t = b.LX.t;
+        r.l_dp = b.LX.dp;
+        r.l_ps = lam_ps;
+        r.hyai = c.hyai_d;
+        r.hybi = c.hybi_d;
+        r.ps0 = c.ps0;
+        r.scr = b.rscr;
+        launch_adj_remap(c.st, r);
+      }
+    } else {
       copy_state(b.LX, b.L0, f3);  // adjoint of its output = adjoint of the next step's input
     }
-    for (int cyc = nsub - 1; cyc >= 0; --cyc) hv_adjoint(H[cyc], H[cyc + 1], b.LX, dt / nsub);
+    for (int cyc = nsub - 1; cyc >= 0; --cyc) hv_adjoint(b, H[cyc], H[cyc + 1], b.LX, dt / nsub);
     launch_adj_zero(c.st, 4 * f3, b.L0.v);
     launch_adj_zero(c.st, 4 * f3, b.L1.v);
     // stage 5 from w = (5 u1 - u0)/4 and u4; then stages 4..1
-    caar_adjoint(U[4], b.LX, 3.0 * dt / 4.0, b.LX, false, b.L1, 1.25, &b.L0, -0.25);
-    caar_adjoint(U[3], b.LX, 2.0 * dt / 3.0, b.LX, false, b.L0, 1.0, nullptr, 0.0);
-    caar_adjoint(U[2], b.LX, dt / 3.0, b.LX, false, b.L0, 1.0, nullptr, 0.0);
-    caar_adjoint(U[1], b.LX, dt / 5.0, b.L1, true, b.L0, 1.0, nullptr, 0.0);
-    caar_adjoint(U[0], b.L1, dt / 5.0, b.L0, true, b.L0, 1.0, nullptr, 0.0);
+    caar_adjoint(b, U[4], b.LX, 3.0 * dt / 4.0, b.LX, false, b.L1, 1.25, &b.L0, -0.25);
+    caar_adjoint(b, U[3], b.LX, 2.0 * dt / 3.0, b.LX, false, b.L0, 1.0, nullptr, 0.0);
+    caar_adjoint(b, U[2], b.LX, dt / 3.0, b.LX, false, b.L0, 1.0, nullptr, 0.0);
+    caar_adjoint(b, U[1], b.LX, dt / 5.0, b.L1, true, b.L0, 1.0, nullptr, 0.0);
+    caar_adjoint(b, U[0], b.L1, dt / 5.0, b.L0, true, b.L0, 1.0, nullptr, 0.0);
   }
   hipMemcpyAsync(lam_v, b.L0.v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
   hipMemcpyAsync(lam_T, b.L0.t, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
   launch_adj_ps(c.st, c.nelem, c.nlev, c.hybi_d, b.L0.dp, lam_ps);  // dp3d = dhyai ps0 + dhybi ps
+}
+
+// The adjoint step on the buffers b (adj_layout for nd dynamics steps): the
+// forward pass keeps each dynamics step's input and the last step's stage
+// states; the reverse sweep recomputes the stage states of the others.
+int adj_step_run(AdjBufs &b, int nd, double dt, const double *v, const double *T, const double *ps, double *lam_v,
+                 double *lam_T, double *lam_ps, const char *who) {
+  Ctx &c = C();
+  {
+    Scope s("adj_forward");
+    adj_forward_steps(b, nd, dt, v, T, ps);
+  }
+  Scope s("adj_reverse");
+  auto stages = [&](int i) -> const Stages & {
+    if (i < nd - 1) adj_dyn_forward(b, i, dt);
+    return b.cur;
+  };
+  adj_reverse_call(b, nd, dt, stages, lam_v, lam_T, lam_ps);
   return hipok(hipStreamSynchronize(c.st), who) ? 0 : 4;
+}
+
+// stage states for nsets dynamics steps (hommexx_hip_adjoint_segment)
+bool seg_alloc(int nsets) {
+  Ctx &c = C();
+  const size_t f3 = (size_t)c.nelem * c.nlev * 16;
+  const int nsub = c.p.hypervis_subcycle;
+  if (segb && segb->f3 == f3 && segb->nsub == nsub && segb->nsets >= nsets) return true;
+  delete segb;
+  segb = new SegBufs();
+  SegBufs &g = *segb;
+  const int per = 5 + 1 + nsub + 1;
+  double *q = dalloc<double>(((size_t)nsets * per + 1) * 4 * f3 + (size_t)c.nelem * 16);
+  if (!q) {
+    delete segb;
+    segb = nullptr;
+    return false;
+  }
+  g.f3 = f3;
+  g.nsub = nsub;
+  g.nsets = nsets;
+  g.sets.resize(nsets);
+  for (Stages &st : g.sets) {
+    for (int i = 0; i < 5; ++i) st.U.push_back(carve_state(q, f3));
+    st.W = carve_state(q, f3);
+    for (int i = 0; i <= nsub; ++i) st.H.push_back(carve_state(q, f3));
+  }
+  g.x = carve_state(q, f3);
+  g.ps = q;
+  return true;
 }
 
 }  // namespace
@@ -1687,6 +1751,77 @@ int hommexx_hip_adjoint_step(double dt, const double *v, const double *T, const 
 
 
 
+// The adjoint of n consecutive prim_run_subcycle_c calls of length dt (a
+// checkpointed segment), from the state (v, T, ps) before the first: the
+// forward pass keeps the stage states of every dynamics step (40 F3 each at
+// hypervis_subcycle 3), so the reverse sweep recomputes nothing.  lam_* as
+// for hommexx_hip_adjoint_step (output adjoint in, input adjoint out).
+int hommexx_hip_adjoint_segment(double dt, int n, const double *v, const double *T, const double *ps, double *lam_v,
+                                double *lam_T, double *lam_ps) {
+  Ctx &c = C();
+  if (!adj_supported("hommexx_hip_adjoint_segment")) return 2;
+  if (n < 1) {
+    fail(73, "hommexx_hip_adjoint_segment: %d calls", n);
+    return 2;
+  }
+  const int nd = c.p.qsplit * (c.p.rsplit > 0 ? c.p.rsplit : 1);
+  if (!adj_alloc(nd) || !seg_alloc(n * nd)) return 3;
+  AdjBufs &b = *adjb;
+  SegBufs &g = *segb;
+  const size_t f3 = b.f3;
+  const int nsub = c.p.hypervis_subcycle;
+  {
+    Scope s("adj_forward");
+    launch_zero_int(c.st, c.err_d);
+    hipMemcpyAsync(g.x.v, v, 2 * f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+    hipMemcpyAsync(g.x.t, T, f3 * sizeof(double), hipMemcpyDeviceToDevice, c.st);
+    launch_dp3d_from_ps(c.st, c.nelem, c.nlev, c.hyai_d, c.hybi_d, c.ps0, ps, g.x.dp);
+    for (int call = 0; call < n; ++call) {
+      for (int i = 0; i < nd; ++i)
+        dyn_forward_stages(i == 0 ? g.x : g.sets[call * nd + i - 1].H[nsub], g.sets[call * nd + i], dt);
+      if (call + 1 == n) break;
+      // the next call's input: remapped v, T (rsplit > 0), ps from the
+      // thicknesses and dp3d from ps, as prim_run_subcycle_c leaves them
+      copy_state(g.x, g.sets[call * nd + nd - 1].H[nsub], f3);
+      if (c.p.rsplit > 0) {
+        RemapArgs a{};
+        a.nelem = c.nelem;
+        a.nlev = c.nlev;
+        a.qsize = 0;
+        a.nstates = 3;
+        a.remap_alg = c.p.remap_alg;
+        a.v = g.x.v;
+        a.t = g.x.t;
+        a.dp = g.x.dp;
+        a.ps = g.ps;
+        a.hyai = c.hyai_d;
+        a.hybi = c.hybi_d;
+        a.ps0 = c.ps0;
+        a.hyai0 = c.hyai0;
+        a.error = c.err_d;
+        launch_remap(c.st, a);
+      } else {
+        launch_ps_from_dp(c.st, c.nelem, c.nlev, c.hyai0, c.ps0, g.x.dp, g.ps);
+      }
+      launch_dp3d_from_ps(c.st, c.nelem, c.nlev, c.hyai_d, c.hybi_d, c.ps0, g.ps, g.x.dp);
+    }
+  }
+  {
+    Scope s("adj_reverse");
+    for (int call = n - 1; call >= 0; --call) {
+      auto stages = [&](int i) -> const Stages & { return g.sets[call * nd + i]; };
+      adj_reverse_call(b, nd, dt, stages, lam_v, lam_T, lam_ps);
+    }
+  }
+  hipMemcpyAsync(c.err_h, c.err_d, sizeof(int), hipMemcpyDeviceToHost, c.st);
+  if (!hipok(hipStreamSynchronize(c.st), "hommexx_hip_adjoint_segment")) return 4;
+  if (*c.err_h & 1) {
+    fail(60, "Negative (or nan) layer thickness detected, aborting!");
+    return 5;
+  }
+  return 0;
+}
+
 // The map hommexx_hip_adjoint_step differentiates, forward: (v, T, ps) is
 // advanced in place by the dynamics steps of one prim_run_subcycle_c call and
 // the remap of v and T (rsplit > 0), with no tracer step; for a dry run the
@@ -1701,7 +1836,7 @@ int hommexx_hip_dynamics_step(double dt, double *v, double *T, double *ps) {
   const size_t f3 = b.f3;
   launch_zero_int(c.st, c.err_d);
   adj_forward_steps(b, nd, dt, v, T, ps);
-  const StateBuf &o = b.H[c.p.hypervis_subcycle];
+  const StateBuf &o = b.cur.H[c.p.hypervis_subcycle];
   if (c.p.rsplit > 0) {
     RemapArgs a{};
     a.nelem = c.nelem;

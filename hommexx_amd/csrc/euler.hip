@@ -68,9 +68,9 @@ __global__ __launch_bounds__(256) void euler_divdp_qmm_kernel(EulerArgs a) {
     const double x = nx / dp;
     nx = a.qdp_n0[q0i + (size_t)(q + 1 < Q ? q + 1 : q) * qs];
     const double mn = slab_min(x), mx = slab_max(x);
-    double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
-    ql[k] = mn;
-    ql[L + k] = mx;
+    double *ql = a.qlim + ((size_t)ie * Q + q) * ql_row(L);
+    ql[ql_min(k)] = mn;
+    ql[ql_min(k) + 8] = mx;
   };
   // the first tracer is peeled so that on both paths into the loop the same
   // stores follow the prefetch (the wait then covers the prefetch only)
@@ -89,19 +89,19 @@ __global__ __launch_bounds__(256) void euler_nbr_minmax_kernel(EulerArgs a) {
   const int k = (int)(i % L);
   const int ie = (int)((i / L) % a.nelem);
   const int q = (int)(i / ((size_t)L * a.nelem));
-  const size_t own = (((size_t)ie * Q + q) * 2) * L + k;
-  double mn = a.qlim[own], mx = a.qlim[own + L];
+  const size_t own = ((size_t)ie * Q + q) * ql_row(L) + ql_min(k);
+  double mn = a.qlim[own], mx = a.qlim[own + 8];
   // an absent neighbour (-1) reads the element's own row, which leaves the
   // min and max unchanged: no branch, all loads in flight together
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int nb = a.nbr[ie * 8 + c];
-    const size_t o = opaque_index((((size_t)(nb < 0 ? ie : nb) * Q + q) * 2) * L + k);
-    mx = dmax(mx, a.qlim[o + L]);
+    const size_t o = opaque_index(((size_t)(nb < 0 ? ie : nb) * Q + q) * ql_row(L) + ql_min(k));
+    mx = dmax(mx, a.qlim[o + 8]);
     mn = dmin(mn, a.qlim[o]);
   }
   a.qlim_new[own] = mn;
-  a.qlim_new[own + L] = mx;
+  a.qlim_new[own + 8] = mx;
 }
 
 // gather of point (ie, pt) at level k from a pre-exchange field (layout
@@ -284,9 +284,9 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
   auto fetch = [&](int q) {
     const int qq = q < Q ? q : Q - 1;
     const size_t qi = (((size_t)ie * Q + qq) * L + kc) * 16 + pt;
-    const double *ql = a.qlim + (((size_t)ie * Q + qq) * 2) * L;
-    pf[2] = ql[kc];
-    pf[3] = ql[L + kc];
+    const double *ql = a.qlim + ((size_t)ie * Q + qq) * ql_row(L);
+    pf[2] = ql[ql_min(kc)];
+    pf[3] = ql[ql_min(kc) + 8];
     pf[0] = a.qdp_n0[qi];
     pf[1] = qbp[qi];
     if (MIXED) pf[4] = qbf[bihf ? qi : 0];
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
   if (PF && Q > 0) fetch(0);
   for (int q = 0; q < Q; ++q) {
     const size_t qi = (((size_t)ie * Q + q) * L + kc) * 16 + pt;
-    double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
+    double *ql = a.qlim + ((size_t)ie * Q + q) * ql_row(L);
     double minp, maxp, qd, q0pf = 0.0;
     if (PF) {
       qd = pf[0];
@@ -306,8 +306,8 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
       // this tracer's bounds are loaded before the next tracer's gather is
       // issued: loads complete in issue order, so waiting for them does not
       // wait for the gather
-      minp = ql[kc];
-      maxp = ql[L + kc];
+      minp = ql[ql_min(kc)];
+      maxp = ql[ql_min(kc) + 8];
     }
     if (GATHER) {
       // the stage-1 exchange of tracer q; tracer q+1's loads are issued
@@ -364,8 +364,8 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
       }
     }
     if (pt == 0 && son) {
-      if (wmin) ql[k] = minp;
-      if (wmax) ql[L + k] = maxp;
+      if (wmin) ql[ql_min(k)] = minp;
+      if (wmax) ql[ql_min(k) + 8] = maxp;
     }
     if (son) a.qdp_pre[((size_t)ie * Q + q) * fs + lc_index(pt, k)] = e.sph * ptens;
   }
@@ -486,9 +486,9 @@ __global__ __launch_bounds__(256) void euler_qmm_lap_kernel(EulerArgs a) {
     nx = a.qdp_n0[q0i + (size_t)(q + 1 < Q ? q + 1 : q) * qs];
     const double mn = slab_min(x), mx = slab_max(x);
     if (pt == 0) {
-      double *ql = a.qlim + (((size_t)ie * Q + q) * 2) * L;
-      ql[k] = mn;
-      ql[L + k] = mx;
+      double *ql = a.qlim + ((size_t)ie * Q + q) * ql_row(L);
+      ql[ql_min(k)] = mn;
+      ql[ql_min(k) + 8] = mx;
     }
     const double y = scale ? x * wdp / dp0 : x;
     const size_t o = ((size_t)ie * Q + q) * lc_field(L) + lc_index(pt, k);

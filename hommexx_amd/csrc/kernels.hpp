@@ -62,6 +62,11 @@ struct HvArgs {
 __host__ __device__ inline size_t lc_field(int nlev) { return (size_t)((nlev + 7) >> 3) * 128; }
 __host__ __device__ inline size_t lc_index(int pt, int k) { return (size_t)(k >> 3) * 128 + pt * 8 + (k & 7); }
 
+// Tracer bounds qlim: per element and tracer, levels in chunks of 8 with a
+// chunk's minima and maxima in one 128-byte line, [ie][q][lev/8][min,max][lev%8].
+__host__ __device__ inline size_t ql_row(int nlev) { return (size_t)((nlev + 7) >> 3) * 16; }
+__host__ __device__ inline size_t ql_min(int k) { return (size_t)(k >> 3) * 16 + (k & 7); }
+
 struct EulerArgs {
   MetricPtrs m;
   const int *src, *nbr;
@@ -70,7 +75,7 @@ struct EulerArgs {
   double *qdp_pre;            // pre-DSS output
   double *qdp_np1;            // final output (DSS)
   double *qbih, *qbih_pre;    // biharmonic buffers
-  double *qlim, *qlim_new;    // [ie][q][2][lev]
+  double *qlim, *qlim_new;    // ql_row / ql_min layout
   const double *derived_dp, *divdp, *divdp_proj, *vn0, *dpdiss_ave, *dpdiss_bih, *dp0;
   double *divdp_w, *divdp_proj_w;
   double *fdss, *fdss_pre;    // the extra DSS field and its pre-exchange copy

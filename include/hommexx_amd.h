@@ -162,6 +162,14 @@ int hommexx_hip_get_field(int which, int q, double *dst);
 int hommexx_hip_set_state(const double *v, const double *T, const double *ps);
 int hommexx_hip_adjoint_step(double dt, const double *v, const double *T, const double *ps, double *lam_v,
                              double *lam_T, double *lam_ps);
+/* the adjoint of n consecutive steps of length dt (one checkpointed segment)
+ * from the state (v, T, ps) before the first; lam_* as for
+ * hommexx_hip_adjoint_step.  The forward pass keeps the stage states of every
+ * dynamics step of the segment (40 3-D fields each at hypervis_subcycle 3),
+ * so nothing is recomputed in the reverse sweep.  Same conditions; the
+ * session's model state is not changed.  Returns 0 on success. */
+int hommexx_hip_adjoint_segment(double dt, int n, const double *v, const double *T, const double *ps, double *lam_v,
+                                double *lam_T, double *lam_ps);
 /* the map hommexx_hip_adjoint_step differentiates, forward: (v, T, ps) is
  * advanced in place by one prim_run_subcycle_c step of length dt without its
  * tracer step (for a dry run the (v, T, ps) that step produces); used to

@@ -77,6 +77,30 @@ def test_dynamics_step_is_the_forward_step(rsplit):
                 assert torch.equal(a, b), (rsplit, n)
 
 
+@pytest.mark.parametrize("rsplit,qsplit", [(0, 1), (1, 1), (3, 1), (1, 2)])
+def test_adjoint_segment_is_the_chained_step(rsplit, qsplit):
+    """hommexx_hip_adjoint_segment over 4 steps (stage states kept) gives bit
+    for bit what hommexx_hip_adjoint_step gives step by step on the states
+    that hommexx_hip_dynamics_step replays."""
+    prob = build_problem(ne=4, nlev=26, qsize=2, seed=35, rsplit=rsplit, qsplit=qsplit, t_noise=0.1)
+    with DiffSession(prob) as s:
+        x0 = initial_state(prob)
+        states = [x0]
+        for _ in range(3):
+            y = tuple(t.clone() for t in states[-1])
+            s.dynamics_step(y)
+            states.append(y)
+        gen = torch.Generator(device="cuda").manual_seed(7)
+        lam = [torch.randn(a.shape, generator=gen, dtype=torch.float64, device="cuda") for a in x0]
+        a = [t.clone() for t in lam]
+        for st in reversed(states):
+            s.adjoint_step(st, a)
+        b = [t.clone() for t in lam]
+        s.adjoint_segment(x0, 4, b)
+    for p, q in zip(a, b):
+        assert torch.equal(p, q), (rsplit, qsplit)
+
+
 @pytest.mark.parametrize("rsplit", [0, 1, 3])
 def test_adjoint_checkpoint_interval_independent(rsplit):
     """The gradient does not depend on the checkpoint interval (every = 1
