@@ -24,6 +24,9 @@
 
 using namespace hx;
 
+// the array behind scratch_pointer() (see there): 16 Mi doubles, 128 MiB per GPU
+constexpr size_t ADJ_SCRATCH_DOUBLES = (size_t)16 << 20;
+static __device__ __align__(256) double scratch[ADJ_SCRATCH_DOUBLES];
 
 namespace {
 
@@ -676,6 +679,30 @@ StateBuf carve_state(double *&q, size_t f3) {
   return s;
 }
 
+// Fixed device scratch of hommexx_hip_adjoint_step_noalloc: ADJ_SCRATCH_DOUBLES
+// (128 MiB) per GPU, held until the process ends.  In host code `scratch`
+// only names the symbol: its device address comes from hipGetSymbolAddress,
+// once per device with that device current, and only that address goes to
+// kernels and copies.
+std::map<int, double *> scratch_ptrs;  // device -> the address of its copy of scratch
+
+bool scratch_setup() {
+  int dev = 0;
+  if (!hipok(hipGetDevice(&dev), "hipGetDevice")) return false;
+  if (scratch_ptrs.count(dev)) return true;
+  void *p = nullptr;
+  if (!hipok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return false;
+  scratch_ptrs[dev] = (double *)p;
+  return true;
+}
+
+double *scratch_pointer() {
+  int dev = 0;
+  if (!hipok(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+  if (!scratch_ptrs.count(dev) && !scratch_setup()) return nullptr;
+  return scratch_ptrs[dev];
+}
+
 // doubles of the adjoint step's buffers for nd dynamics steps per call
 size_t adj_total_doubles(int nd) {
   Ctx &c = C();
@@ -1146,6 +1173,7 @@ void initialize_hommexx_session() {
     return;
   }
   if (!hipok(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking), "hipStreamCreate")) return;
+  if (!scratch_setup()) return;  // the no-allocation adjoint step's array on this device
   c.session = true;
 }
 
@@ -1748,7 +1776,38 @@ int hommexx_hip_adjoint_step(double dt, const double *v, const double *T, const 
   return adj_step_run(*adjb, nd, dt, v, T, ps, lam_v, lam_T, lam_ps, "hommexx_hip_adjoint_step");
 }
 
-
+// As hommexx_hip_adjoint_step, but every buffer of the step comes from the
+// library's fixed device array `scratch` (ADJ_SCRATCH_DOUBLES doubles, one
+// per GPU, held until the process ends) instead of device allocations: for
+// callers that cannot allocate (graph capture, serving loops).  Larger
+// configurations are rejected (error 71).  All calls on one GPU share its
+// array, so calls that overlap on one GPU race.
+int hommexx_hip_adjoint_step_noalloc(double dt, const double *v, const double *T, const double *ps, double *lam_v,
+                                     double *lam_T, double *lam_ps) {
+  Ctx &c = C();
+  if (!ready("hommexx_hip_adjoint_step_noalloc")) return 2;
+  if (!c.be_set) {
+    fail(72, "hommexx_hip_adjoint_step_noalloc: init_boundary_exchanges_c has not run (it allocates)");
+    return 2;
+  }
+  if (!adj_supported("hommexx_hip_adjoint_step_noalloc")) return 2;
+  const int nd = c.p.qsplit * (c.p.rsplit > 0 ? c.p.rsplit : 1);
+  const size_t need = adj_total_doubles(nd);
+  if (need > ADJ_SCRATCH_DOUBLES) {
+    fail(71, "hommexx_hip_adjoint_step_noalloc: needs %zu doubles of scratch, the library holds %zu", need,
+         (size_t)ADJ_SCRATCH_DOUBLES);
+    return 3;
+  }
+  double *base = scratch_pointer();
+  if (!base) return 3;
+  static AdjBufs nb;  // the layout is rebuilt only when the configuration changes
+  if (nb.base != base || nb.f3 != (size_t)c.nelem * c.nlev * 16 || nb.nsub != c.p.hypervis_subcycle || nb.nd != nd)
+    adj_layout(nb, base, nd);
+  // the array keeps the last call's contents: start from zeros, as the
+  // allocated buffers of hommexx_hip_adjoint_step do
+  launch_adj_zero(c.st, need, base);
+  return adj_step_run(nb, nd, dt, v, T, ps, lam_v, lam_T, lam_ps, "hommexx_hip_adjoint_step_noalloc");
+}
 
 
 // The adjoint of n consecutive prim_run_subcycle_c calls of length dt (a

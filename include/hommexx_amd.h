@@ -170,6 +170,17 @@ int hommexx_hip_adjoint_step(double dt, const double *v, const double *T, const 
  * session's model state is not changed.  Returns 0 on success. */
 int hommexx_hip_adjoint_segment(double dt, int n, const double *v, const double *T, const double *ps, double *lam_v,
                                 double *lam_T, double *lam_ps);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving or training loops).
+ * hommexx_hip_adjoint_step with every buffer taken from one fixed device
+ * array of 16 Mi doubles (128 MiB) that the library holds on each GPU until
+ * the process ends; it calls no hipMalloc or hipFree.  Configurations that
+ * need more (about ne4 at 72 levels is the limit) fail with error 71 (return
+ * 3); init_boundary_exchanges_c must have run.  All calls on one GPU share
+ * its array: calls that overlap on one GPU race.  Same results as
+ * hommexx_hip_adjoint_step. */
+int hommexx_hip_adjoint_step_noalloc(double dt, const double *v, const double *T, const double *ps, double *lam_v,
+                                     double *lam_T, double *lam_ps);
 /* the map hommexx_hip_adjoint_step differentiates, forward: (v, T, ps) is
  * advanced in place by one prim_run_subcycle_c step of length dt without its
  * tracer step (for a dry run the (v, T, ps) that step produces); used to
