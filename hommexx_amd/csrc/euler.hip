@@ -17,6 +17,8 @@
 //   advect(3)  ; dss + SSP average (extra field omega_p)
 // Limiter sums run over the 16 points in index order (slab_sum_serial) to
 // round like the reference's serial reductions.
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace hx {
@@ -197,6 +199,7 @@ constexpr int AGL = 8;
 template <bool MIXED, bool GATHER>
 __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stage) {
   __shared__ double tile[GATHER ? 2 : 1][AGL][17];
+  __shared__ double ssum[GATHER ? 16 * AGL : 1];  // the serial sums of the gathering kernel
   size_t slab;
   int ie, k, pt;
   const int L = a.nlev, Q = a.qsize;
@@ -267,8 +270,13 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
   const double c = e.sph * dpdissk;
   const double s1 = limit ? slab_sum_serial(c) : 0.0;  // same for every tracer
   const size_t fs = lc_field(L), es = (size_t)Q * fs;
-  double graw[4] = {};  // the loads of the next tracer's gather
-  if (GATHER && Q > 0) gather_lf_load(a.qdp_pre_in, es, 0, ie, gpt, gkc, g0, g1, g2, graw);
+  // the tracer pre-exchange values are stored in FP32 in the mixed form (the
+  // exchange sums them in FP64)
+  using PreT = typename std::conditional<MIXED, float, double>::type;
+  const PreT *qpin = (const PreT *)a.qdp_pre_in;
+  PreT *qpout = (PreT *)a.qdp_pre;
+  PreT graw[4] = {};  // the loads of the next tracer's gather
+  if (GATHER && Q > 0) gather_lf_load(qpin, es, 0, ie, gpt, gkc, g0, g1, g2, graw);
   const int kc = son ? k : L - 1;  // reads of a padding slab stay in bounds
   // Slab path (stages 1 and 3): tracer q+1's Qdp, biharmonic term and
   // bounds are loaded during tracer q's arithmetic.  The loads are
@@ -317,7 +325,7 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
       if (gon) tile[q & 1][gkk][gpt] = y;
       // (unconditional, the last tracer reloads itself: a branch here would
       // make every later wait cover the gathers)
-      gather_lf_load(a.qdp_pre_in, es, (size_t)(q + 1 < Q ? q + 1 : q) * fs, ie, gpt, gkc, g0, g1, g2, graw);
+      gather_lf_load(qpin, es, (size_t)(q + 1 < Q ? q + 1 : q) * fs, ie, gpt, gkc, g0, g1, g2, graw);
       lds_barrier();
       qd = tile[q & 1][kk][pt];
     } else if (!PF) {
@@ -344,7 +352,26 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
     bool wmin = stage == 2, wmax = stage == 2;
     if (limit) {
       double x = ptens / dpdissk;
-      const double s0 = slab_sum_serial(x * c);
+      double s0;
+      if (GATHER) {
+        // the 16 values of the slab go through LDS and every lane of the
+        // slab adds them in index order: fewer VALU instructions than the
+        // DPP chain, which measured faster in this (VALU-bound) kernel and
+        // slower in the slab kernels; the rounding is the same
+        __builtin_amdgcn_wave_barrier();
+        ssum[threadIdx.x] = x * c;
+        __builtin_amdgcn_wave_barrier();
+        const double *row = ssum + (threadIdx.x & ~15u);
+        s0 = row[0];
+#pragma unroll
+        for (int j = 1; j < 16; ++j) {
+          s0 += row[j];
+          if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+        }
+        __builtin_amdgcn_wave_barrier();
+      } else {
+        s0 = slab_sum_serial(x * c);
+      }
       if (!(s1 <= 0)) {
         if (minp < 0) {
           minp = 0;
@@ -367,7 +394,7 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
       if (wmin) ql[ql_min(k)] = minp;
       if (wmax) ql[ql_min(k) + 8] = maxp;
     }
-    if (son) a.qdp_pre[((size_t)ie * Q + q) * fs + lc_index(pt, k)] = e.sph * ptens;
+    if (son) qpout[((size_t)ie * Q + q) * fs + lc_index(pt, k)] = (PreT)(e.sph * ptens);
   }
 }
 
@@ -410,7 +437,9 @@ __device__ __forceinline__ bool xchg_lane(int nelem, int nlev, XLane &x) {
 
 // DSS of tracers [QB*blockIdx.y, QB*blockIdx.y + QB) and, in the first batch,
 // of the extra field.  avg: stage 3's SSP average (Qdp(n0) + 2 Qdp(np1)) / 3
-// (:379-403) of the assembled tracer value.
+// (:379-403) of the assembled tracer value.  F32: the tracer pre-exchange
+// values are FP32 (mixed form), summed in FP64.
+template <bool F32>
 __global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int avg) {
   __shared__ double tile[2][XL][17];
   XLane x;
@@ -428,7 +457,9 @@ __global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int avg) {
   if (nq > 0) {
 #pragma unroll
     for (int i = 0; i < QB; ++i)
-      v[i] = gather_lf(a.qdp_pre, es, (size_t)(q0 + (i < nq ? i : 0)) * fs, x.ie, x.pt, kc, s0, s1, s2);
+      v[i] = F32 ? gather_lf((const float *)a.qdp_pre, es, (size_t)(q0 + (i < nq ? i : 0)) * fs, x.ie, x.pt, kc, s0,
+                             s1, s2)
+                 : gather_lf(a.qdp_pre, es, (size_t)(q0 + (i < nq ? i : 0)) * fs, x.ie, x.pt, kc, s0, s1, s2);
   }
   if (q0 == 0) gather_lf_load(a.fdss_pre, fs, 0, x.ie, x.pt, kc, s0, s1, s2, fr);
 #pragma unroll
@@ -588,7 +619,9 @@ void launch_euler_advect_gather(hipStream_t st, const EulerArgs &a) {
 
 void launch_euler_dss(hipStream_t st, const EulerArgs &a, int avg) {
   const int nq = a.qsize > 0 ? (a.qsize + QB - 1) / QB : 1;
-  hipLaunchKernelGGL(euler_dss_kernel, dim3(xchg_blocks_x(a.nelem, a.nlev), nq), dim3(256), 0, st, a, avg);
+  const dim3 g(xchg_blocks_x(a.nelem, a.nlev), nq);
+  if (a.mixed) hipLaunchKernelGGL(euler_dss_kernel<true>, g, dim3(256), 0, st, a, avg);
+  else hipLaunchKernelGGL(euler_dss_kernel<false>, g, dim3(256), 0, st, a, avg);
 }
 
 void launch_euler_qmm_lap(hipStream_t st, const EulerArgs &a) {

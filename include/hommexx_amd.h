@@ -121,9 +121,11 @@ void hommexx_hip_dimensions(int *nlev, int *qsize_d);
 void hommexx_hip_set_abort_on_error(int on);
 /* 0 (default): the FP64 step.  1: the mixed-precision form -- the sphere
  * operators of hyperviscosity, of the tracer flux divergence and of the
- * tracer biharmonic run in FP32 with compensated contractions; column scans,
- * remap, DSS accumulation, limiters and RK combinations stay FP64.  Takes
- * effect at the next prim_run_subcycle_c.  Returns 1 for an unknown form. */
+ * tracer biharmonic run in FP32 with compensated contractions, and the
+ * tracer values handed from one kernel to the next exchange (pre-exchange
+ * buffers) are stored in FP32; the prognostic state, column scans, remap,
+ * DSS accumulation, limiters and RK combinations stay FP64.  Takes effect at
+ * the next prim_run_subcycle_c.  Returns 1 for an unknown form. */
 int hommexx_hip_set_form(int form);
 /* debugging aid: prim_run_subcycle_c returns after phase n (1: first RK stage,
  * 2: RK stages, 3: hyperviscosity, 4: first tracer stage, 5: tracers), 0: off */

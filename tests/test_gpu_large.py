@@ -93,4 +93,8 @@ def test_large_mixed(runs, name):
                                                ", ".join("%s %.2e" % kv for kv in err.items())))
     for k, e in err.items():
         assert e <= 1e-6, (name, k, e)
-    assert dd <= 1e-13 and dq <= 1e-11
+    # the dry mass is conserved to FP64 rounding; the tracer pre-exchange
+    # values are rounded to FP32 before each exchange, which perturbs the
+    # global tracer mass by the sum of those roundings (measured 4.3e-10 over
+    # the 200 ne60 steps, 5.4e-11 over 10 ne120 steps)
+    assert dd <= 1e-13 and dq <= 5e-9
