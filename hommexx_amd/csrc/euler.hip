@@ -61,13 +61,14 @@ __global__ __launch_bounds__(256) void euler_divdp_qmm_kernel(EulerArgs a) {
   a.divdp_proj_w[s] = dv;
   const double rhsmdt = a.rhs_multiplier * a.dt;
   const double dp = a.derived_dp[s] - rhsmdt * dv;
+  const Recip rdp = recip(dp);  // Qdp / dp for every tracer
   // tracer q+1's Qdp is loaded before tracer q's reductions; every lane
   // stores the slab's bounds (the same value: measured faster here than a
   // store from one lane, whose branch makes the next wait cover the stores)
   const size_t q0i = ((size_t)ie * Q * L + k) * 16 + pt, qs = (size_t)L * 16;
   double nx = Q > 0 ? a.qdp_n0[q0i] : 0.0;
   auto tracer = [&](int q) {
-    const double x = nx / dp;
+    const double x = div_r(nx, rdp);
     nx = a.qdp_n0[q0i + (size_t)(q + 1 < Q ? q + 1 : q) * qs];
     const double mn = slab_min(x), mx = slab_max(x);
     double *ql = a.qlim + ((size_t)ie * Q + q) * ql_row(L);
@@ -106,26 +107,43 @@ __global__ __launch_bounds__(256) void euler_nbr_minmax_kernel(EulerArgs a) {
   a.qlim_new[own + 8] = mx;
 }
 
-// gather of point (ie, pt) at level k from a pre-exchange field (layout
-// lc_index) with element stride es (own value, then the sources in the
-// reference's order); gather_lf_load issues the four loads, gather_sum adds
-// them (FP32 buffers of the mixed form are accumulated in FP64 the same way)
+// gather of point (ie, pt) at level k from pre-exchange fields (layout
+// lc_index) with element stride es: the own copy and the three sources'
+// copies (an absent source points at the own copy), in the reference's
+// order.  The four addresses are formed once per lane; field f of a batch is
+// then one offset (f * field size, uniform) added to each.  load issues the
+// four loads, gather_sum adds them (FP32 buffers of the mixed form are
+// accumulated in FP64 the same way).
 template <typename T>
-__device__ __forceinline__ void gather_lf_load(const T *f, size_t es, size_t off, int ie, int pt, int k, int s0,
-                                               int s1, int s2, T (&r)[4]) {
-  const size_t own = (size_t)ie * es + off + lc_index(pt, k);
-  auto at = [&](int s) { return opaque_index(s >= 0 ? (size_t)(s >> 4) * es + off + lc_index(s & 15, k) : own); };
-  r[0] = f[own];
-  r[1] = f[at(s0)];
-  r[2] = f[at(s1)];
-  r[3] = f[at(s2)];
-}
+struct LfGather {
+  const T *p[4];
+  __device__ __forceinline__ LfGather(const T *f, size_t es, int ie, int pt, int k, int s0, int s1, int s2) {
+    p[0] = f + (size_t)ie * es + lc_index(pt, k);
+    // (opaque: the selected address stays one unconditional load)
+    auto at = [&](int s) { return opaque_ptr(s >= 0 ? f + (size_t)(s >> 4) * es + lc_index(s & 15, k) : p[0]); };
+    p[1] = at(s0);
+    p[2] = at(s1);
+    p[3] = at(s2);
+  }
+  __device__ __forceinline__ void load(size_t off, T (&r)[4]) const {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r[j] = p[j][off];
+  }
+  __device__ __forceinline__ double sum(size_t off, int s0, int s1, int s2) const {
+    T r[4];
+    load(off, r);
+    return gather_sum(r, s0, s1, s2);
+  }
+};
 
+// The same gather with the addresses formed per field: fewer live registers
+// than LfGather's four pointers (euler_dss keeps 8 waves per SIMD with it).
 template <typename T>
 __device__ __forceinline__ double gather_lf(const T *f, size_t es, size_t off, int ie, int pt, int k, int s0, int s1,
                                             int s2) {
-  T r[4];
-  gather_lf_load(f, es, off, ie, pt, k, s0, s1, s2, r);
+  const size_t own = (size_t)ie * es + off + lc_index(pt, k);
+  auto at = [&](int s) { return opaque_index(s >= 0 ? (size_t)(s >> 4) * es + off + lc_index(s & 15, k) : own); };
+  const T r[4] = {f[own], f[at(s0)], f[at(s1)], f[at(s2)]};
   return gather_sum(r, s0, s1, s2);
 }
 
@@ -142,7 +160,9 @@ __device__ __forceinline__ void limit8_slab(double mass, double minp, double max
       delta = x - minp;
       x = minp;
     }
-    const double addmass = slab_any(delta != 0.0) ? slab_sum_serial(delta * c) : 0.0;
+    // (skipped when no slab of the wave clips; a slab that clips nothing
+    // in a wave that does sums zeros: exactly 0, as if skipped)
+    const double addmass = wave_any(delta != 0.0) ? slab_sum_serial(delta * c) : 0.0;
     if (fabs(addmass) <= tol_limiter * fabs(mass)) break;
     if (addmass > 0) {
       const double weightssum = slab_sum_serial(x < maxp ? c : 0.0);
@@ -166,7 +186,7 @@ __device__ __forceinline__ void limit9_slab(double minp, double maxp, double &x,
     delta = x - minp;
     x = minp;
   }
-  const double addmass = slab_any(delta != 0.0) ? slab_sum_serial(delta * c) : 0.0;
+  const double addmass = wave_any(delta != 0.0) ? slab_sum_serial(delta * c) : 0.0;
   if (addmass == 0) return;
   if (addmass > 0) {
     double fac = slab_sum_serial(c * (maxp - x));
@@ -196,10 +216,11 @@ __device__ __forceinline__ void limit9_slab(double minp, double maxp, double &x,
 // stored, later stages read it.
 constexpr int AGL = 8;
 
+// (at most 96 VGPRs: five waves per SIMD)
 template <bool MIXED, bool GATHER>
-__global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stage) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void euler_advect_kernel(EulerArgs a,
+                                                                                                     int stage) {
   __shared__ double tile[GATHER ? 2 : 1][AGL][17];
-  __shared__ double ssum[GATHER ? 16 * AGL : 1];  // the serial sums of the gathering kernel
   size_t slab;
   int ie, k, pt;
   const int L = a.nlev, Q = a.qsize;
@@ -239,7 +260,7 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
   const int gkc = gon ? gk : L - 1;  // gathers of a padding lane stay in bounds
   if (GATHER) {
     // assembled divdp_proj of stage 1
-    const double y = gather_lf(a.fdss_pre_in, lc_field(L), 0, ie, gpt, gkc, g0, g1, g2) * gr;
+    const double y = LfGather<double>(a.fdss_pre_in, lc_field(L), ie, gpt, gkc, g0, g1, g2).sum(0, g0, g1, g2) * gr;
     if (gon) tile[0][gkk][gpt] = y;
     lds_barrier();
     dpproj = tile[0][kk][pt];
@@ -268,6 +289,8 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
     rmd = (float)(1.0 / e.metdet * RREARTH);
   }
   const double c = e.sph * dpdissk;
+  // the divisors shared by the tracers: Qdp / dp (stage 2), ptens / dpdissk
+  const Recip rdp = recip(dp), rdpd = recip(dpdissk);
   const double s1 = limit ? slab_sum_serial(c) : 0.0;  // same for every tracer
   const size_t fs = lc_field(L), es = (size_t)Q * fs;
   // the tracer pre-exchange values are stored in FP32 in the mixed form (the
@@ -276,7 +299,8 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
   const PreT *qpin = (const PreT *)a.qdp_pre_in;
   PreT *qpout = (PreT *)a.qdp_pre;
   PreT graw[4] = {};  // the loads of the next tracer's gather
-  if (GATHER && Q > 0) gather_lf_load(qpin, es, 0, ie, gpt, gkc, g0, g1, g2, graw);
+  const LfGather<PreT> gq(qpin, es, ie, gpt, gkc, g0, g1, g2);
+  if (GATHER && Q > 0) gq.load(0, graw);
   const int kc = son ? k : L - 1;  // reads of a padding slab stay in bounds
   // Slab path (stages 1 and 3): tracer q+1's Qdp, biharmonic term and
   // bounds are loaded during tracer q's arithmetic.  The loads are
@@ -289,20 +313,25 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
   const double *qbp = add_hv && !a.bih32 ? a.qbih : a.qdp_n0;
   const float *qbf = (const float *)(bihf ? a.qbih : a.qdp_n0);
   double pf[5] = {};  // Qdp, biharmonic term (FP64, FP32), qmin, qmax of the next tracer
+  // the lane's addresses for tracer 0; tracer q adds q times a (uniform)
+  // tracer stride
+  const size_t qs = (size_t)L * 16, q0i = ((size_t)ie * Q * L + kc) * 16 + pt;
+  const double *qdp_l = a.qdp_n0 + q0i, *qbp_l = qbp + q0i;
+  const float *qbf_l = bihf ? qbf + q0i : qbf;
+  double *ql_l = a.qlim + (size_t)ie * Q * ql_row(L) + ql_min(kc);
+  PreT *qpo_l = qpout + (size_t)ie * es + lc_index(pt, kc);
   auto fetch = [&](int q) {
-    const int qq = q < Q ? q : Q - 1;
-    const size_t qi = (((size_t)ie * Q + qq) * L + kc) * 16 + pt;
-    const double *ql = a.qlim + ((size_t)ie * Q + qq) * ql_row(L);
-    pf[2] = ql[ql_min(kc)];
-    pf[3] = ql[ql_min(kc) + 8];
-    pf[0] = a.qdp_n0[qi];
-    pf[1] = qbp[qi];
-    if (MIXED) pf[4] = qbf[bihf ? qi : 0];
+    const size_t qq = q < Q ? q : Q - 1;
+    pf[2] = ql_l[qq * ql_row(L)];
+    pf[3] = ql_l[qq * ql_row(L) + 8];
+    pf[0] = qdp_l[qq * qs];
+    pf[1] = qbp_l[qq * qs];
+    if (MIXED) pf[4] = qbf_l[bihf ? qq * qs : 0];
   };
   if (PF && Q > 0) fetch(0);
   for (int q = 0; q < Q; ++q) {
-    const size_t qi = (((size_t)ie * Q + q) * L + kc) * 16 + pt;
-    double *ql = a.qlim + ((size_t)ie * Q + q) * ql_row(L);
+    const size_t qi = (size_t)q * qs;  // tracer q's offset from qdp_l, qbp_l, qbf_l
+    double *ql = ql_l + (size_t)q * ql_row(L);
     double minp, maxp, qd, q0pf = 0.0;
     if (PF) {
       qd = pf[0];
@@ -314,8 +343,8 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
       // this tracer's bounds are loaded before the next tracer's gather is
       // issued: loads complete in issue order, so waiting for them does not
       // wait for the gather
-      minp = ql[ql_min(kc)];
-      maxp = ql[ql_min(kc) + 8];
+      minp = ql[0];
+      maxp = ql[8];
     }
     if (GATHER) {
       // the stage-1 exchange of tracer q; tracer q+1's loads are issued
@@ -325,14 +354,14 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
       if (gon) tile[q & 1][gkk][gpt] = y;
       // (unconditional, the last tracer reloads itself: a branch here would
       // make every later wait cover the gathers)
-      gather_lf_load(qpin, es, (size_t)(q + 1 < Q ? q + 1 : q) * fs, ie, gpt, gkc, g0, g1, g2, graw);
+      gq.load((size_t)(q + 1 < Q ? q + 1 : q) * fs, graw);
       lds_barrier();
       qd = tile[q & 1][kk][pt];
     } else if (!PF) {
-      qd = a.qdp_n0[qi];
+      qd = qdp_l[qi];
     }
     if (stage == 2) {  // compute_qmin_qmax with rhs_multiplier == 1
-      const double xq = qd / dp;
+      const double xq = div_r(qd, rdp);
       minp = dmin(minp, slab_min(xq));
       maxp = dmax(maxp, slab_max(xq));
     }
@@ -340,7 +369,7 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
     if (MIXED) {
       // flux divergence in FP32 with compensated contractions; the update
       // qdp - dt*div + qbih stays FP64
-      const double q0 = PF ? q0pf : (add_hv ? (a.bih32 ? (double)((const float *)a.qbih)[qi] : a.qbih[qi]) : 0);
+      const double q0 = PF ? q0pf : (add_hv ? (bihf ? (double)qbf_l[qi] : qbp_l[qi]) : 0);
       const double fa = vs0 * qd, fb = vs1 * qd;
       const float gv0 = (float)((e.Dinv[0] * fa + e.Dinv[2] * fb) * e.metdet);
       const float gv1 = (float)((e.Dinv[1] * fa + e.Dinv[3] * fb) * e.metdet);
@@ -351,27 +380,8 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
     }
     bool wmin = stage == 2, wmax = stage == 2;
     if (limit) {
-      double x = ptens / dpdissk;
-      double s0;
-      if (GATHER) {
-        // the 16 values of the slab go through LDS and every lane of the
-        // slab adds them in index order: fewer VALU instructions than the
-        // DPP chain, which measured faster in this (VALU-bound) kernel and
-        // slower in the slab kernels; the rounding is the same
-        __builtin_amdgcn_wave_barrier();
-        ssum[threadIdx.x] = x * c;
-        __builtin_amdgcn_wave_barrier();
-        const double *row = ssum + (threadIdx.x & ~15u);
-        s0 = row[0];
-#pragma unroll
-        for (int j = 1; j < 16; ++j) {
-          s0 += row[j];
-          if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
-        }
-        __builtin_amdgcn_wave_barrier();
-      } else {
-        s0 = slab_sum_serial(x * c);
-      }
+      double x = div_r(ptens, rdpd);
+      const double s0 = slab_sum_serial(x * c);
       if (!(s1 <= 0)) {
         if (minp < 0) {
           minp = 0;
@@ -391,10 +401,10 @@ __global__ __launch_bounds__(256) void euler_advect_kernel(EulerArgs a, int stag
       }
     }
     if (pt == 0 && son) {
-      if (wmin) ql[ql_min(k)] = minp;
-      if (wmax) ql[ql_min(k) + 8] = maxp;
+      if (wmin) ql[0] = minp;
+      if (wmax) ql[8] = maxp;
     }
-    if (son) qpout[((size_t)ie * Q + q) * fs + lc_index(pt, k)] = (PreT)(e.sph * ptens);
+    if (son) qpo_l[(size_t)q * fs] = (PreT)(e.sph * ptens);
   }
 }
 
@@ -454,14 +464,13 @@ __global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int avg) {
   const int kc = gk ? x.k : L - 1;  // gathers of a padding lane stay in bounds
   // every load of the block's gathers is issued before the first sum
   double v[QB], fr[4];
+  using PreT = typename std::conditional<F32, float, double>::type;
   if (nq > 0) {
 #pragma unroll
     for (int i = 0; i < QB; ++i)
-      v[i] = F32 ? gather_lf((const float *)a.qdp_pre, es, (size_t)(q0 + (i < nq ? i : 0)) * fs, x.ie, x.pt, kc, s0,
-                             s1, s2)
-                 : gather_lf(a.qdp_pre, es, (size_t)(q0 + (i < nq ? i : 0)) * fs, x.ie, x.pt, kc, s0, s1, s2);
+      v[i] = gather_lf((const PreT *)a.qdp_pre, es, (size_t)(q0 + (i < nq ? i : 0)) * fs, x.ie, x.pt, kc, s0, s1, s2);
   }
-  if (q0 == 0) gather_lf_load(a.fdss_pre, fs, 0, x.ie, x.pt, kc, s0, s1, s2, fr);
+  if (q0 == 0) LfGather<double>(a.fdss_pre, fs, x.ie, x.pt, kc, s0, s1, s2).load(0, fr);
 #pragma unroll
   for (int i = 0; i < QB; ++i) {
     if (i >= nq) break;
@@ -471,7 +480,7 @@ __global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int avg) {
     if (wkv) {
       const size_t o = (((size_t)x.ie * Q + q0 + i) * L + x.k0 + x.wk) * 16 + x.wpt;
       double y = t[x.wk][x.wpt];
-      if (avg) y = (a.qdp_avg_n0[o] + (3.0 - 1) * y) / 3.0;
+      if (avg) y = div_c<3>(a.qdp_avg_n0[o] + (3.0 - 1) * y);
       st_stream(&a.qdp_np1[o], y);
     }
   }
@@ -501,6 +510,7 @@ __global__ __launch_bounds__(256) void euler_qmm_lap_kernel(EulerArgs a) {
   const double dp = a.derived_dp[s] - rhsmdt * a.divdp_proj[s];
   const double wdp = a.dpdiss_ave[s];
   const double dp0 = a.dp0[k];
+  const Recip rdp = recip(dp), rdp0 = recip(dp0);  // shared by the tracers
   ElemLaneF ef;
   DvvLaneF df;
   if (MIXED) {
@@ -513,7 +523,7 @@ __global__ __launch_bounds__(256) void euler_qmm_lap_kernel(EulerArgs a) {
   const bool scale = a.nu_p > 0;
   double nx = a.qdp_n0[q0i];
   auto tracer = [&](int q) {
-    const double x = nx / dp;
+    const double x = div_r(nx, rdp);
     nx = a.qdp_n0[q0i + (size_t)(q + 1 < Q ? q + 1 : q) * qs];
     const double mn = slab_min(x), mx = slab_max(x);
     if (pt == 0) {
@@ -521,7 +531,7 @@ __global__ __launch_bounds__(256) void euler_qmm_lap_kernel(EulerArgs a) {
       ql[ql_min(k)] = mn;
       ql[ql_min(k) + 8] = mx;
     }
-    const double y = scale ? x * wdp / dp0 : x;
+    const double y = scale ? div_r(x * wdp, rdp0) : x;
     const size_t o = ((size_t)ie * Q + q) * lc_field(L) + lc_index(pt, k);
     if (MIXED) ((float *)a.qbih_pre)[o] = sph_laplacef(ef, df, (float)y);
     else a.qbih_pre[o] = sph_laplace(e, d, y);
@@ -552,11 +562,11 @@ __global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
   const int kc = gk ? x.k : L - 1;  // gathers of a padding lane stay in bounds
   // every load of the block's gathers is issued before the first sum
   double v[QB];
+  {
+    using PreT = typename std::conditional<MIXED, float, double>::type;
+    const LfGather<PreT> g((const PreT *)a.qbih_pre, es, x.ie, x.pt, kc, s0, s1, s2);
 #pragma unroll
-  for (int i = 0; i < QB; ++i) {
-    const size_t off = (size_t)(q0 + (i < nq ? i : 0)) * fs;
-    if (MIXED) v[i] = gather_lf((const float *)a.qbih_pre, es, off, x.ie, x.pt, kc, s0, s1, s2);
-    else v[i] = gather_lf(a.qbih_pre, es, off, x.ie, x.pt, kc, s0, s1, s2);
+    for (int i = 0; i < QB; ++i) v[i] = g.sum((size_t)(q0 + (i < nq ? i : 0)) * fs, s0, s1, s2);
   }
   ElemLane e;
   load_elem_light(a.m, x.ie, x.wpt, e);
@@ -572,6 +582,7 @@ __global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
   if (TENSOR) load_tensor(a.m, x.ie, x.wpt, tv, vs);
   const double f = -a.rhs_viss * a.dt * a.nu_q;
   const double dp0 = wkv ? a.dp0[wk] : 0.0;
+  const Recip rsph = recip(e.sph);  // shared by the tracers
 #pragma unroll
   for (int i = 0; i < QB; ++i) {
     if (i >= nq) break;
@@ -583,14 +594,14 @@ __global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
     const size_t o = (((size_t)x.ie * Q + q0 + i) * L + wk) * 16 + x.wpt;
     if (MIXED) {
       const double lap = sph_laplacef(ef, df, (float)y);
-      if (wkv) st_stream(&((float *)a.qbih)[o], (float)(f * dp0 * lap / e.sph));
+      if (wkv) st_stream(&((float *)a.qbih)[o], (float)div_r(f * dp0 * lap, rsph));
     } else if (TENSOR) {
       // BIHPostTensorHV (EulerStepFunctorImpl.hpp:281-291)
       const double lap = sph_laplace_tensor(e, d, tv, y);
-      if (wkv) st_stream(&a.qbih[o], f * dp0 * lap / e.sph);
+      if (wkv) st_stream(&a.qbih[o], div_r(f * dp0 * lap, rsph));
     } else {
       const double lap = sph_laplace(e, d, y);
-      if (wkv) st_stream(&a.qbih[o], f * dp0 * lap / e.sph);
+      if (wkv) st_stream(&a.qbih[o], div_r(f * dp0 * lap, rsph));
     }
   }
 }

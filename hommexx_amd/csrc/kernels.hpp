@@ -217,12 +217,17 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// An index the compiler cannot see through: the selected source index of a
-// gather stays one unconditional load (otherwise the select of addresses is
-// turned back into a branch around the neighbour load and a wait).
+// An index (or address) the compiler cannot see through: the selected source
+// of a gather stays one unconditional load (otherwise the select of addresses
+// is turned back into a branch around the neighbour load and a wait).
 __device__ __forceinline__ size_t opaque_index(size_t i) {
   asm volatile("" : "+v"(i));
   return i;
+}
+template <typename T>
+__device__ __forceinline__ T *opaque_ptr(T *p) {
+  asm volatile("" : "+v"(p));
+  return p;
 }
 
 // DSS gather of one point: the own value r[0] and up to three neighbour

@@ -5,9 +5,10 @@
 // the fastest index (Fortran i), igp the slower one (Fortran j).
 //   * row_get<K>(x): value at (igp, K) of the caller's slab  -> DPP quad_perm
 //   * col_get<K>(x): value at (K, jgp) of the caller's slab  -> ds_swizzle
-//   * slab_sum_serial(x): ((x0 + x1) + x2) + ... + x15, left to right, in
-//     every lane of the slab (systolic DPP row_shr:1 chain, then broadcast
-//     of lane 15) -- the same rounding as a serial loop over the points.
+//   * slab_sum_serial(x): (((0 + x0) + x1) + x2) + ... + x15, left to right,
+//     in every lane of the slab (64-bit DPP row_newbcast: each step adds
+//     point j's value to every lane of the row) -- the same rounding as a
+//     serial loop over the points.
 //   * slab_min / slab_max: exact, any order.
 // All operations act on doubles as two 32-bit halves.
 #pragma once
@@ -51,35 +52,39 @@ __device__ __forceinline__ double col_get(double x) {
   return mk64(h, l);
 }
 
-// value of lane 15 of each 16-lane row, broadcast to the row
-__device__ __forceinline__ double row_last(double x) {
-  constexpr int pattern = 0x10 | (0x0F << 5);
-  const int l = __builtin_amdgcn_ds_swizzle(lo32(x), pattern);
-  const int h = __builtin_amdgcn_ds_swizzle(hi32(x), pattern);
-  return mk64(h, l);
-}
-
-// row_shr:1 within 16-lane rows, lane 0 of each row receives 0.0
-__device__ __forceinline__ double shr1(double x) { return mk64(dpp_mov<0x111>(hi32(x)), dpp_mov<0x111>(lo32(x))); }
-
-// Serial (left-to-right over points 0..15) sum, result in every lane of the slab.
+// Serial (left-to-right over points 0..15) sum, result in every lane of the
+// slab: s = 0, then s = fma(x_j, 1, s) for j = 0..15, where x_j is lane j of
+// the 16-lane row broadcast by row_newbcast:j (the one DPP control 64-bit
+// VALU operations take on CDNA3/4).  fma(x, 1, s) rounds once, exactly as
+// s + x.  The leading s_nop covers the two wait states a DPP source needs
+// after a VALU write; the trailing one covers the same for a DPP read of the
+// result by code the compiler schedules after this block.
 __device__ __forceinline__ double slab_sum_serial(double v) {
-  double t = v;
-#pragma unroll
-  for (int j = 0; j < 15; ++j) t = shr1(t) + v;
-  return row_last(t);
-}
-
-// Two independent serial sums interleaved (more ILP).
-__device__ __forceinline__ void slab_sum_serial2(double v, double w, double &sv, double &sw) {
-  double t = v, u = w;
-#pragma unroll
-  for (int j = 0; j < 15; ++j) {
-    t = shr1(t) + v;
-    u = shr1(u) + w;
-  }
-  sv = row_last(t);
-  sw = row_last(u);
+  double s;
+  const double one = 1.0;
+  asm volatile(
+      "v_mov_b64 %0, 0\n\t"
+      "s_nop 1\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:4 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:5 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:6 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:7 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:8 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:9 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:10 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:11 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:12 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:13 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:14 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:15 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1"
+      : "=&v"(s)
+      : "v"(v), "v"(one));
+  return s;
 }
 
 // xor-butterfly within 16 lanes via ds_swizzle (and_mask 0x1F, xor_mask M)
@@ -91,30 +96,63 @@ __device__ __forceinline__ double xor_get(double x) {
   return mk64(h, l);
 }
 
+// Division by a divisor shared by many quotients, the same value as a / b
+// (bit for bit, except that a zero quotient may come back as +0): with y = 1/b correctly rounded, q0 = a*y is within one ulp of a/b, the
+// remainder a - q0*b is exact in one FMA, and q0 + rem*y rounded once is the
+// correctly rounded quotient (Markstein's theorem on FMA division, IBM J.
+// Res. Dev. 34(1), 1990), for finite operands whose remainder does not
+// underflow (|a| far above 2^-960).  Three VALU operations per quotient
+// instead of the ten of an IEEE division.
+struct Recip {
+  double b, y;
+};
+__device__ __forceinline__ Recip recip(double b) { return {b, 1.0 / b}; }
+__device__ __forceinline__ double div_r(double a, const Recip &r) {
+  const double q0 = a * r.y;
+  const double rem = __builtin_fma(-q0, r.b, a);
+  return __builtin_fma(rem, r.y, q0);
+}
+// the same with a constant divisor (y folded by the compiler)
+template <int B>
+__device__ __forceinline__ double div_c(double a) {
+  return div_r(a, Recip{(double)B, 1.0 / (double)B});
+}
+
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 
+// One v_min_f64 / v_max_f64: fmin/fmax of a swizzled value would first be
+// quieted by a canonicalising v_max_f64 (the operands here are never NaN).
+// Of two equal values of opposite sign either zero may come back; the
+// value is the same.
+__device__ __forceinline__ double vmin64(double a, double b) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ double vmax64(double a, double b) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 __device__ __forceinline__ double slab_min(double x) {
-  x = dmin(x, xor_get<1>(x));
-  x = dmin(x, xor_get<2>(x));
-  x = dmin(x, xor_get<4>(x));
-  x = dmin(x, xor_get<8>(x));
+  x = vmin64(x, xor_get<1>(x));
+  x = vmin64(x, xor_get<2>(x));
+  x = vmin64(x, xor_get<4>(x));
+  x = vmin64(x, xor_get<8>(x));
   return x;
 }
 __device__ __forceinline__ double slab_max(double x) {
-  x = dmax(x, xor_get<1>(x));
-  x = dmax(x, xor_get<2>(x));
-  x = dmax(x, xor_get<4>(x));
-  x = dmax(x, xor_get<8>(x));
+  x = vmax64(x, xor_get<1>(x));
+  x = vmax64(x, xor_get<2>(x));
+  x = vmax64(x, xor_get<4>(x));
+  x = vmax64(x, xor_get<8>(x));
   return x;
 }
 
-// true if any lane of the caller's slab has pred set
-__device__ __forceinline__ bool slab_any(bool pred) {
-  const uint64_t b = __ballot(pred);
-  const int shift = (__lane_id() & 48);
-  return ((b >> shift) & 0xFFFFull) != 0;
-}
+// true if any active lane of the wave has pred set (wave-uniform)
+__device__ __forceinline__ bool wave_any(bool pred) { return __ballot(pred) != 0; }
 
 // ---------------------------------------------------------------- metric
 // Per-lane constants of one element: the 2x2 tensors at the lane's point in

@@ -53,8 +53,9 @@ __device__ __forceinline__ double dmin3(double a, double b, double c) {
   return a < bc ? a : bc;
 }
 
+// (the divisions by 3 and 6 below are div_c: the same value, fewer operations)
 __device__ __forceinline__ double integrate_parabola(double sq, double lin, double cst, double x1, double x2) {
-  return (cst * (x2 - x1) + lin * (x2 * x2 - x1 * x1) / 2.0) + sq * (x2 * x2 * x2 - x1 * x1 * x1) / 3.0;
+  return (cst * (x2 - x1) + lin * (x2 * x2 - x1 * x1) / 2.0) + div_c<3>(sq * (x2 * x2 * x2 - x1 * x1 * x1));
 }
 
 // dma(j) from a(j-1..j+1) and the grid factors g0, g1, g2 of dx(j-1..j+1)
@@ -82,11 +83,11 @@ __device__ __forceinline__ void ppm_parabola(double al, double ar, double a, dou
   // the two monotonicity tests share (ar - al) and (ar - al)^2 / 6 unless
   // the first one moves al (same operations and rounding as evaluating each
   // test in full)
-  double d = ar - al, q = d * d / 6.0;
+  double d = ar - al, q = div_c<6>(d * d);
   if (d * (a - (al + ar) / 2.0) > q) {
     al = 3.0 * a - 2.0 * ar;
     d = ar - al;
-    q = d * d / 6.0;
+    q = div_c<6>(d * d);
   }
   if (d * (a - (al + ar) / 2.0) < -q) ar = 3.0 * a - 2.0 * al;
   c0 = 1.5 * a - (al + ar) / 4.0;
@@ -106,9 +107,9 @@ __device__ __forceinline__ void ppm_parabola(double al, double ar, double a, dou
 //   ppmdx rows for dma(j+1) and ai(j), dx(j) and the incoming dx(j+3)),
 //   iteration j0 + row computed by row row.
 constexpr int RCH = 16;  // sweep iterations per grid chunk (rows 0..RCH-1 compute one each)
-constexpr int RG = 10;   // shared grid values per (iteration, column)
+constexpr int RG = 11;   // shared grid values per (iteration, column)
 
-size_t remap_lds_bytes(int nlev) { return (size_t)RCH * RG * 16 * 8 + (size_t)nlev * (16 * (8 + 2) + 2 * 8); }
+size_t remap_lds_bytes(int nlev) { return (size_t)RCH * RG * 16 * 8 + (size_t)nlev * (16 * (8 + 8 + 2) + 2 * 8); }
 
 template <int RFB>  // fields per lane
 __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
@@ -116,7 +117,8 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
   const int L = a.nlev;
   double *gsh = rsm;                              // [RCH][RG][16]
   double *zsh = rsm + RCH * RG * 16;              // z2  [L][16]
-  double *dab = zsh + (size_t)L * 16;             // hyai(k+1) - hyai(k), hybi(k+1) - hybi(k)  [L][2]
+  double *rtk = zsh + (size_t)L * 16;             // 1 / (target thickness)  [L][16]
+  double *dab = rtk + (size_t)L * 16;             // hyai(k+1) - hyai(k), hybi(k+1) - hybi(k)  [L][2]
   short *ksh = (short *)(dab + (size_t)L * 2);    // kid [L][16]
   const int ie = xcd_block(blockIdx.x, gridDim.x);
   const int row = threadIdx.x >> 4, pt = threadIdx.x & 15;
@@ -190,6 +192,10 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
   if (row == 0) {
     if (lsrc) search([&](int c) { return gsh[c * 16 + pt]; });
     else search([&](int c) { return src[(size_t)c * 16]; });
+  } else {
+    // meanwhile the other rows form the reciprocals of the target
+    // thicknesses (the outputs' divisor, shared by the fields)
+    for (int k = row - 1; k < L; k += 15) rtk[k * 16 + pt] = 1.0 / (dab[2 * k] * a.ps0 + dab[2 * k + 1] * ps);
   }
 
   // ------------------------------------------------------------ fields
@@ -279,6 +285,7 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
         g[112] = X1 * (X1 + X2) / (X0 + 2.0 * X1);
         g[128] = X0;
         g[144] = dxf(j + 3);  // the cell entering the window (used for j + 3 <= L)
+        g[160] = 1.0 / g[144];
       }
     }
     __syncthreads();
@@ -324,6 +331,7 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
           }
           const double z2 = zsh[k * 16 + pt];
           const double tk = dab[2 * k] * a.ps0 + dab[2 * k + 1] * ps;  // the reference's (hyai(k+1)-hyai(k))*ps0 + ...
+          const Recip rt{tk, rtk[k * 16 + pt]};
 #pragma unroll
           for (int i = 0; i < RFB; ++i) {
             if (!on[i]) continue;
@@ -332,10 +340,10 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
             const double out = k > 0 ? mass - mass_prev[i] : mass;
             mass_prev[i] = mass;
             if (state[i]) {
-              var[i][(size_t)k * vs[i]] = out / tk;
+              var[i][(size_t)k * vs[i]] = div_r(out, rt);
             } else {
               var[i][(size_t)k * vs[i]] = out;
-              if (qout[i]) ((__attribute__((address_space(1))) double *)Qout[i])[(size_t)k * 16] = out / tk;
+              if (qout[i]) ((__attribute__((address_space(1))) double *)Qout[i])[(size_t)k * 16] = div_r(out, rt);
             }
           }
           ++k;
@@ -347,15 +355,15 @@ __global__ __launch_bounds__(256) void remap_kernel(RemapArgs a) {
       }
       // shift: bring in cell j+3
       const int m = j + 3;
-      const double xn = g[144];
+      const Recip rxn{g[144], g[160]};
 #pragma unroll
       for (int i = 0; i < RFB; ++i) {
         if (!on[i]) continue;
         double an, vn = 0.0;
         if (m <= L) {
           const double x = pv[i];
-          vn = state[i] ? x * xn : x;
-          an = vn / xn;
+          vn = state[i] ? x * rxn.b : x;
+          an = div_r(vn, rxn);
         } else if (m == L + 1) {
           an = A2[i];  // a(L+1) = a(L)
         } else {
