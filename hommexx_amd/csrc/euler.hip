@@ -410,14 +410,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void e
 
 // ---------------------------------------------------------------- exchanges
 // The pre-exchange buffers (qdp_pre, qbih_pre, fdss_pre) are stored levels
-// fastest in chunks of 8 levels (lc_index), so a neighbour copy of 16
-// consecutive levels of one GLL point is two 64-byte runs, each half of a
-// line whose other half the same block reads.  An exchange block is one
-// element x 16 levels (XL): lane (pt, kk) gathers point pt at level k0+kk,
-// then the assembled values go through an LDS tile to lanes (kk, pt) and are
-// stored in the slab layout [ie][q][lev][pt].  Blocks walk the elements of
-// one level group in order, so neighbour runs come from L2.
-constexpr int XL = 16;
+// fastest in chunks of 8 levels (lc_index), so a neighbour copy of the 8
+// levels of one GLL point in a chunk is one 64-byte run, half of a line whose
+// other half the same block reads.  An exchange block is one element x one
+// 8-level chunk (XL; 128 lanes, no level padding when the level count is a
+// multiple of 8): lane (pt, kk) gathers point pt at level k0+kk, then the
+// assembled values go through an LDS tile to lanes (kk, pt) and are stored in
+// the slab layout [ie][q][lev][pt].  Blocks walk the elements of one level
+// group in order, so neighbour runs come from L2.
+constexpr int XL = 8;
 constexpr int QB = 8;  // tracers per exchange block
 
 inline int xchg_blocks_x(int nelem, int nlev) {
@@ -436,8 +437,8 @@ __device__ __forceinline__ bool xchg_lane(int nelem, int nlev, XLane &x) {
   const int g = lb / nelem;
   x.k0 = g * XL;
   if (x.k0 >= nlev) return false;  // whole block
-  x.pt = threadIdx.x >> 4;
-  x.kk = threadIdx.x & 15;
+  x.pt = threadIdx.x / XL;
+  x.kk = threadIdx.x % XL;
   x.k = x.k0 + x.kk;
   x.wk = threadIdx.x >> 4;
   x.wpt = threadIdx.x & 15;
@@ -631,8 +632,8 @@ void launch_euler_advect_gather(hipStream_t st, const EulerArgs &a) {
 void launch_euler_dss(hipStream_t st, const EulerArgs &a, int avg) {
   const int nq = a.qsize > 0 ? (a.qsize + QB - 1) / QB : 1;
   const dim3 g(xchg_blocks_x(a.nelem, a.nlev), nq);
-  if (a.mixed) hipLaunchKernelGGL(euler_dss_kernel<true>, g, dim3(256), 0, st, a, avg);
-  else hipLaunchKernelGGL(euler_dss_kernel<false>, g, dim3(256), 0, st, a, avg);
+  if (a.mixed) hipLaunchKernelGGL(euler_dss_kernel<true>, g, dim3(16 * XL), 0, st, a, avg);
+  else hipLaunchKernelGGL(euler_dss_kernel<false>, g, dim3(16 * XL), 0, st, a, avg);
 }
 
 void launch_euler_qmm_lap(hipStream_t st, const EulerArgs &a) {
@@ -643,9 +644,9 @@ void launch_euler_qmm_lap(hipStream_t st, const EulerArgs &a) {
 void launch_euler_bih_post(hipStream_t st, const EulerArgs &a) {
   if (a.qsize <= 0) return;
   const dim3 g(xchg_blocks_x(a.nelem, a.nlev), (a.qsize + QB - 1) / QB);
-  if (a.bih32) hipLaunchKernelGGL((euler_bih_post_kernel<true, false>), g, dim3(256), 0, st, a);
-  else if (a.consthv) hipLaunchKernelGGL((euler_bih_post_kernel<false, false>), g, dim3(256), 0, st, a);
-  else hipLaunchKernelGGL((euler_bih_post_kernel<false, true>), g, dim3(256), 0, st, a);
+  if (a.bih32) hipLaunchKernelGGL((euler_bih_post_kernel<true, false>), g, dim3(16 * XL), 0, st, a);
+  else if (a.consthv) hipLaunchKernelGGL((euler_bih_post_kernel<false, false>), g, dim3(16 * XL), 0, st, a);
+  else hipLaunchKernelGGL((euler_bih_post_kernel<false, true>), g, dim3(16 * XL), 0, st, a);
 }
 
 }  // namespace hx
