@@ -452,7 +452,8 @@ __device__ __forceinline__ bool xchg_lane(int nelem, int nlev, XLane &x) {
 // values are FP32 (mixed form), summed in FP64.
 template <bool F32>
 __global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int avg) {
-  __shared__ double tile[2][XL][17];
+  // every assembled field of the block goes into LDS before one barrier
+  __shared__ double tile[QB + 1][XL][17];
   XLane x;
   if (!xchg_lane(a.nelem, a.nlev, x)) return;
   const int L = a.nlev, Q = a.qsize, q0 = blockIdx.y * QB;
@@ -472,24 +473,29 @@ __global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int avg) {
       v[i] = gather_lf((const PreT *)a.qdp_pre, es, (size_t)(q0 + (i < nq ? i : 0)) * fs, x.ie, x.pt, kc, s0, s1, s2);
   }
   if (q0 == 0) LfGather<double>(a.fdss_pre, fs, x.ie, x.pt, kc, s0, s1, s2).load(0, fr);
+  if (gk) {
 #pragma unroll
-  for (int i = 0; i < QB; ++i) {
-    if (i >= nq) break;
-    double (*t)[17] = tile[i & 1];
-    if (gk) t[x.kk][x.pt] = v[i] * r;
-    lds_barrier();
-    if (wkv) {
-      const size_t o = (((size_t)x.ie * Q + q0 + i) * L + x.k0 + x.wk) * 16 + x.wpt;
-      double y = t[x.wk][x.wpt];
-      if (avg) y = div_c<3>(a.qdp_avg_n0[o] + (3.0 - 1) * y);
-      st_stream(&a.qdp_np1[o], y);
-    }
+    for (int i = 0; i < QB; ++i)
+      if (i < nq) tile[i][x.kk][x.pt] = v[i] * r;
+    if (q0 == 0) tile[QB][x.kk][x.pt] = gather_sum(fr, s0, s1, s2) * r;
   }
-  if (q0 == 0) {
-    lds_barrier();
-    if (gk) tile[0][x.kk][x.pt] = gather_sum(fr, s0, s1, s2) * r;
-    lds_barrier();
-    if (wkv) st_stream(&a.fdss[((size_t)x.ie * L + x.k0 + x.wk) * 16 + x.wpt], tile[0][x.wk][x.wpt]);
+  // stage 3: the Qdp(n0) values of the average are loaded before the barrier
+  const size_t o0 = (((size_t)x.ie * Q + q0) * L + x.k0 + x.wk) * 16 + x.wpt, os = (size_t)L * 16;
+  double av[QB];
+  if (avg && wkv && nq > 0) {
+#pragma unroll
+    for (int i = 0; i < QB; ++i) av[i] = a.qdp_avg_n0[o0 + (size_t)(i < nq ? i : 0) * os];
+  }
+  lds_barrier();
+  if (wkv) {
+#pragma unroll
+    for (int i = 0; i < QB; ++i) {
+      if (i >= nq) break;
+      double y = tile[i][x.wk][x.wpt];
+      if (avg) y = div_c<3>(av[i] + (3.0 - 1) * y);
+      st_stream(&a.qdp_np1[o0 + (size_t)i * os], y);
+    }
+    if (q0 == 0) st_stream(&a.fdss[((size_t)x.ie * L + x.k0 + x.wk) * 16 + x.wpt], tile[QB][x.wk][x.wpt]);
   }
 }
 
@@ -549,7 +555,7 @@ __global__ __launch_bounds__(256) void euler_qmm_lap_kernel(EulerArgs a) {
 // 16-lane groups are slabs (one level, 16 points) for the laplacian.
 template <bool MIXED, bool TENSOR>
 __global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
-  __shared__ double tile[2][XL][17];
+  __shared__ double tile[QB][XL][17];  // every tracer of the block, one barrier
   XLane x;
   if (!xchg_lane(a.nelem, a.nlev, x)) return;
   const int L = a.nlev, Q = a.qsize, q0 = blockIdx.y * QB;
@@ -584,14 +590,17 @@ __global__ __launch_bounds__(256) void euler_bih_post_kernel(EulerArgs a) {
   const double f = -a.rhs_viss * a.dt * a.nu_q;
   const double dp0 = wkv ? a.dp0[wk] : 0.0;
   const Recip rsph = recip(e.sph);  // shared by the tracers
+  if (gk) {
+#pragma unroll
+    for (int i = 0; i < QB; ++i)
+      if (i < nq) tile[i][x.kk][x.pt] = v[i];
+  }
+  lds_barrier();
 #pragma unroll
   for (int i = 0; i < QB; ++i) {
     if (i >= nq) break;
-    double (*t)[17] = tile[i & 1];
-    if (gk) t[x.kk][x.pt] = v[i];
-    lds_barrier();
     // every lane of a slab takes part in the laplacian's cross-lane reads
-    const double y = wkv ? t[x.wk][x.wpt] * e.rsmp : 0.0;
+    const double y = wkv ? tile[i][x.wk][x.wpt] * e.rsmp : 0.0;
     const size_t o = (((size_t)x.ie * Q + q0 + i) * L + wk) * 16 + x.wpt;
     if (MIXED) {
       const double lap = sph_laplacef(ef, df, (float)y);
