@@ -456,9 +456,10 @@ HvArgs hv_args(const StateBuf &x, double dt, double eta_ave_w) {
 void hyperviscosity(int np1, double dt, double eta_ave_w) {
   Ctx &c = C();
   const HvArgs a = hv_args(level(np1), dt, eta_ave_w);
-  for (int cyc = 0; cyc < c.p.hypervis_subcycle; ++cyc) {
+  const int nsub = c.p.hypervis_subcycle;
+  for (int cyc = 0; cyc < nsub; ++cyc) {
     Scope s("hv_cycle");
-    launch_hv_cycle(c.st, a);
+    launch_hv_cycle(c.st, a, cyc > 0, cyc + 1 < nsub);
   }
 }
 

@@ -224,6 +224,11 @@ __global__ __launch_bounds__(256) void hv_lap2_pre_mixed_kernel(HvArgs a) {
 
 // Second exchange fused with the update: the assembled tensors are used at
 // once by the lane that assembled them (same gather order as dss_kernel).
+// LAP1: the same kernel also forms the first laplacians of the updated state
+// for the next subcycle (what hv_lap1_kernel would compute from the stored
+// values), from the values in registers; they go to vt_b, tt_b, dpt_b, which
+// the caller points at the next subcycle's first-laplacian buffers.
+template <bool LAP1>
 __global__ __launch_bounds__(256) void hv_dss_update_kernel(HvArgs a) {
   int ie, k, pt;
   if (!group_slab(a.nelem, a.nlev, ie, k, pt)) return;
@@ -256,18 +261,32 @@ __global__ __launch_bounds__(256) void hv_dss_update_kernel(HvArgs a) {
   a.v[v1] = w;
   const double tt = (a.dt * bt * r);
   const double heating = vt0 * u + vt1 * w;
-  a.t[i] = a.t[i] + tt - heating / CP;
-  a.dp[i] = (bdp * r);
+  const double tn = a.t[i] + tt - heating / CP, dpn = (bdp * r);
+  a.t[i] = tn;
+  a.dp[i] = dpn;
+  if (LAP1) {
+    ElemLane e;
+    load_elem(a.m, ie, pt, e);
+    DvvLane d;
+    load_dvv(a.m.dvv, pt, d);
+    a.tt_b[i] = sph_laplace(e, d, tn);
+    a.dpt_b[i] = sph_laplace(e, d, dpn);
+    double l0, l1;
+    sph_vlaplace_contra(e, d, a.nu_ratio1, u, w, l0, l1);
+    a.vt_b[v0] = l0;
+    a.vt_b[v1] = l1;
+  }
 }
 
-void launch_hv_cycle(hipStream_t st, const HvArgs &a) {
+void launch_hv_cycle(hipStream_t st, const HvArgs &a, bool lap1_done, bool lap1_next) {
   const size_t nslab = (size_t)a.nelem * a.nlev;
   const int blocks = (int)((nslab + 15) / 16);
   const dim3 gg(group_blocks_x(a.nelem, a.nlev));
   // the first exchange is gathered by hv_lap2_pre, which writes the second
-  // laplacian to the _b buffers; the update exchanges those
+  // laplacian to the _b buffers; the update exchanges those (and, FP64 form,
+  // may form the next subcycle's first laplacians on the way)
   if (a.mixed) hipLaunchKernelGGL(hv_lap1_mixed_kernel, dim3(blocks), dim3(256), 0, st, a);
-  else hipLaunchKernelGGL(hv_lap1_kernel, dim3(blocks), dim3(256), 0, st, a);
+  else if (!lap1_done) hipLaunchKernelGGL(hv_lap1_kernel, dim3(blocks), dim3(256), 0, st, a);
   if (a.mixed) hipLaunchKernelGGL(hv_lap2_pre_mixed_kernel, gg, dim3(256), 0, st, a);
   else if (a.consthv) hipLaunchKernelGGL(hv_lap2_pre_kernel<false>, gg, dim3(256), 0, st, a);
   else hipLaunchKernelGGL(hv_lap2_pre_kernel<true>, gg, dim3(256), 0, st, a);
@@ -275,7 +294,11 @@ void launch_hv_cycle(hipStream_t st, const HvArgs &a) {
   b.vt_a = a.vt_b;
   b.tt_a = a.tt_b;
   b.dpt_a = a.dpt_b;
-  hipLaunchKernelGGL(hv_dss_update_kernel, gg, dim3(256), 0, st, b);
+  b.vt_b = a.vt_a;
+  b.tt_b = a.tt_a;
+  b.dpt_b = a.dpt_a;
+  if (lap1_next && !a.mixed) hipLaunchKernelGGL(hv_dss_update_kernel<true>, gg, dim3(256), 0, st, b);
+  else hipLaunchKernelGGL(hv_dss_update_kernel<false>, gg, dim3(256), 0, st, b);
 }
 
 }  // namespace hx

@@ -169,7 +169,10 @@ void launch_caar(hipStream_t st, const CaarArgs &a);
 void launch_rk_combo(hipStream_t st, int nelem, int nlev, double *v_nm1, double *t_nm1, double *dp_nm1,
                      const double *v_n0, const double *t_n0, const double *dp_n0);
 void launch_dss(hipStream_t st, const DssArgs &a);
-void launch_hv_cycle(hipStream_t st, const HvArgs &a);
+// lap1_done: the first laplacians are already in vt_a, tt_a, dpt_a (formed by
+// the previous subcycle's update); lap1_next: the update forms them for the
+// next subcycle (FP64 form)
+void launch_hv_cycle(hipStream_t st, const HvArgs &a, bool lap1_done = false, bool lap1_next = false);
 void launch_euler_divdp_qmm(hipStream_t st, const EulerArgs &a);
 void launch_euler_nbr_minmax(hipStream_t st, const EulerArgs &a);
 void launch_euler_advect(hipStream_t st, const EulerArgs &a, int stage);
