@@ -344,8 +344,10 @@ struct StateBuf {
   double *v, *t, *dp;
 };
 
-// out = DSS_r( sph*(a + dt*RHS(b)) ): one CAAR stage with its DSS
-void caar_run(const StateBuf &sa, const StateBuf &sb, const StateBuf &out, double dt, double eta_ave_w) {
+// out = DSS_r( sph*(a + dt*RHS(b)) ): one CAAR stage with its DSS.  lap1:
+// the DSS also forms the first hyperviscosity laplacians of out (FP64 form)
+void caar_run(const StateBuf &sa, const StateBuf &sb, const StateBuf &out, double dt, double eta_ave_w,
+              bool lap1 = false) {
   Ctx &c = C();
   CaarArgs a{};
   a.m = metric();
@@ -387,6 +389,14 @@ void caar_run(const StateBuf &sa, const StateBuf &sb, const StateBuf &out, doubl
   x.f[1] = {c.cv + 16, out.v + 16, ev, 32};
   x.f[2] = {c.ct, out.t, es, 16};
   x.f[3] = {c.cdp, out.dp, es, 16};
+  if (lap1) {
+    x.lap1 = 1;
+    x.m = metric();
+    x.lap_v = c.vt_a;
+    x.lap_t = c.tt_a;
+    x.lap_dp = c.dpt_a;
+    x.nu_ratio1 = c.p.nu_ratio1;
+  }
   Scope s("caar_dss");
   launch_dss(c.st, x);
 }
@@ -396,8 +406,15 @@ StateBuf level(int tl) {
   return {c.v[tl], c.t[tl], c.dp[tl]};
 }
 
-void caar_stage(int nm1, int n0, int np1, double dt, double eta_ave_w) {
-  caar_run(level(nm1), level(n0), level(np1), dt, eta_ave_w);
+void caar_stage(int nm1, int n0, int np1, double dt, double eta_ave_w, bool lap1 = false) {
+  caar_run(level(nm1), level(n0), level(np1), dt, eta_ave_w, lap1);
+}
+
+// hyperviscosity's first laplacians are formed by the last RK stage's DSS:
+// FP64 hyperviscosity with at least one subcycle
+bool hv_lap1_fused() {
+  Ctx &c = C();
+  return c.p.hypervis_subcycle > 0 && !(c.form == 1 && c.consthv);
 }
 
 bool stop_here(int phase) {
@@ -417,7 +434,7 @@ void u3_5stage(int nm1, int n0, int np1, double dt, double eta_ave_w) {
     Scope s("rk_combo");
     launch_rk_combo(c.st, c.nelem, c.nlev, c.v[nm1], c.t[nm1], c.dp[nm1], c.v[n0], c.t[n0], c.dp[n0]);
   }
-  caar_stage(nm1, np1, np1, 3.0 * dt / 4.0, 3.0 * eta_ave_w / 4.0);
+  caar_stage(nm1, np1, np1, 3.0 * dt / 4.0, 3.0 * eta_ave_w / 4.0, hv_lap1_fused());
 }
 
 HvArgs hv_args(const StateBuf &x, double dt, double eta_ave_w) {
@@ -457,9 +474,10 @@ void hyperviscosity(int np1, double dt, double eta_ave_w) {
   Ctx &c = C();
   const HvArgs a = hv_args(level(np1), dt, eta_ave_w);
   const int nsub = c.p.hypervis_subcycle;
+  const bool first = hv_lap1_fused();  // the first laplacians came with the last RK stage
   for (int cyc = 0; cyc < nsub; ++cyc) {
     Scope s("hv_cycle");
-    launch_hv_cycle(c.st, a, cyc > 0, cyc + 1 < nsub);
+    launch_hv_cycle(c.st, a, cyc > 0 || first, cyc + 1 < nsub);
   }
 }
 

@@ -13,6 +13,7 @@ namespace hx {
 // One lane = one (element, level, point) for every field of the call: the
 // source table is read once and all the fields' loads are issued before the
 // first sum, so each lane has up to 4*nf independent loads in flight.
+template <bool LAP1>
 __global__ __launch_bounds__(256) void dss_kernel(DssArgs a) {
   int ie, k, pt;
   if (!group_slab(a.nelem, a.nlev, ie, k, pt)) return;
@@ -24,7 +25,7 @@ __global__ __launch_bounds__(256) void dss_kernel(DssArgs a) {
   auto at = [&](const DssField &F, int src, size_t own) {
     return opaque_index(src >= 0 ? (size_t)(src >> 4) * F.estride + (size_t)k * F.lstride + (src & 15) : own);
   };
-  if (a.f32) {
+  if (!LAP1 && a.f32) {
     // FP32 fields of the mixed form, accumulated in FP64
     float r[DSS_MAX_FIELDS][4];
 #pragma unroll
@@ -61,19 +62,38 @@ __global__ __launch_bounds__(256) void dss_kernel(DssArgs a) {
       r[f][3] = F.in[at(F, s2, own)];
     }
   }
+  double y[DSS_MAX_FIELDS];
 #pragma unroll
   for (int f = 0; f < DSS_MAX_FIELDS; ++f) {
     if (f < a.nf) {
       const DssField &F = a.f[f];
       const double x = gather_sum(r[f], s0, s1, s2);
-      F.out[(size_t)ie * F.estride + (size_t)k * F.lstride + pt] = a.rsmp ? x * r0 : x;
+      y[f] = a.rsmp ? x * r0 : x;
+      F.out[(size_t)ie * F.estride + (size_t)k * F.lstride + pt] = y[f];
     }
+  }
+  if (LAP1) {
+    // the first laplacians of hyperviscosity (the work of hv_lap1_kernel on
+    // the stored state), from the assembled values in registers
+    ElemLane e;
+    load_elem(a.m, ie, pt, e);
+    DvvLane d;
+    load_dvv(a.m.dvv, pt, d);
+    const size_t i = ((size_t)ie * a.nlev + k) * 16 + pt, v = ((size_t)ie * a.nlev + k) * 32 + pt;
+    a.lap_t[i] = sph_laplace(e, d, y[2]);
+    a.lap_dp[i] = sph_laplace(e, d, y[3]);
+    double l0, l1;
+    sph_vlaplace_contra(e, d, a.nu_ratio1, y[0], y[1], l0, l1);
+    a.lap_v[v] = l0;
+    a.lap_v[v + 16] = l1;
   }
 }
 
 void launch_dss(hipStream_t st, const DssArgs &a) {
   if (a.nf <= 0) return;
-  hipLaunchKernelGGL(dss_kernel, dim3(group_blocks_x(a.nelem, a.nlev)), dim3(256), 0, st, a);
+  const dim3 g(group_blocks_x(a.nelem, a.nlev));
+  if (a.lap1) hipLaunchKernelGGL(dss_kernel<true>, g, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(dss_kernel<false>, g, dim3(256), 0, st, a);
 }
 
 }  // namespace hx

@@ -38,6 +38,12 @@ struct DssArgs {
   int f32;               // fields hold FP32 values (accumulated in FP64)
   const int *src;        // [ie][16][3] global point ids (ie*16 + pt) or -1
   const double *rsmp;    // multiply by rspheremp if non-null
+  // lap1 (FP64 state fields v0, v1, T, dp3d in f[0..3]): also the first
+  // hyperviscosity laplacians of the assembled state, into lap_v, lap_t, lap_dp
+  int lap1;
+  MetricPtrs m;
+  double *lap_v, *lap_t, *lap_dp;
+  double nu_ratio1;
 };
 
 struct HvArgs {
