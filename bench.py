@@ -28,6 +28,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# the phase whose launches the timed steps time with HIP events (the largest
+# phase of this workload; the phase pass after the timed steps checks that)
+DOMINANT_PHASE = "euler_advect"
 METRIC = "SYPD @ne30 L72 Q40, 1 GPU: FP64 vs mixed (<=1e-6 rel); adjoint/fwd cost"
 
 
@@ -117,18 +120,28 @@ def run_gpu(args, prob, form="fp64"):
         h.run(args.warmup)
         h.synchronize()
         progress("%d warmup steps done" % args.warmup)
-        h.profile(True)
+        # the timed steps carry HIP events around the launches of one phase
+        # only (the dominant one); every phase is timed in a second pass
         h.profile_reset()
+        h.profile(True, only=DOMINANT_PHASE)
         t0 = time.perf_counter()
         h.run(args.steps)
         h.synchronize()
         t1 = time.perf_counter()
         progress("%d timed steps done: %.3f ms/step" % (args.steps, (t1 - t0) / args.steps * 1e3))
         h.profile(False)
+        prof_timed = h.profile_report()
+        h.profile_reset()
+        h.profile(True)
+        h.run(args.steps)
+        h.synchronize()
+        h.profile(False)
         prof = h.profile_report()
+        progress("%d more steps with every phase timed" % args.steps)
         st = h.state()
         nstep, nm1, n0, np1 = h.tl
     sec = (t1 - t0) / args.steps
+    prof = {"timed": prof_timed, "phases": prof}
     # conservation check over warmup + timed steps
     dry0 = float((sph * prob.state["ps_v"][:, 1]).sum())
     dry = float((sph * st["ps_v"][:, n0]).sum())
@@ -282,7 +295,7 @@ def main():
                for k in ("v", "T", "dp3d", "ps_v", "Qdp", "Q")}
         mixed = {"ms_per_step": round(msec * 1e3, 3), "value": round(args.dt / (365.0 * msec), 3),
                  "speedup_vs_fp64": round(sec / msec, 4), "max_rel_l2_vs_fp64": max(err.values()),
-                 "rel_l2_vs_fp64": err, "steps_compared": args.warmup + args.steps,
+                 "rel_l2_vs_fp64": err, "steps_compared": args.warmup + 2 * args.steps,
                  "dry_mass_rel_drift": mdry, "tracer_mass_rel_drift_max": mq}
     adjoint = None
     if not args.no_adjoint:
@@ -290,22 +303,26 @@ def main():
         adjoint = dict(a1)
         adjoint["rsplit0"] = a0
     sypd = args.dt / (365.0 * sec)
-    # dominant kernel by total time
-    kern = {k: v for k, v in prof.items() if kernel_bytes(k, nelem, nlev, q, 1) is not None}
+    # dominant kernel by total time (phase pass); its average launch time from
+    # the timed steps when it is the phase they timed
+    kern = {k: v for k, v in prof["phases"].items() if kernel_bytes(k, nelem, nlev, q, 1) is not None}
     dom = max(kern, key=lambda k: kern[k][1]) if kern else None
     roof = None
     if dom:
-        cnt, tot = kern[dom]
+        timed = dom in prof["timed"]
+        cnt, tot = prof["timed"][dom] if timed else kern[dom]
         avg_s = tot / cnt / 1e3
         byts = kernel_bytes(dom, nelem, nlev, q, cnt)
         ach = byts / avg_s / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, args), "bytes_per_launch": byts,
-                "avg_launch_ms": round(avg_s * 1e3, 4), "launches": cnt}
+                "avg_launch_ms": round(avg_s * 1e3, 4), "launches": cnt,
+                "measured_in": "timed steps" if timed else "phase pass"}
     model = step_model_bytes(nelem, nlev, q)
     step_roof = {"model_bytes_per_step": model, "achieved": round(model / sec / 1e9, 1),
                  "frac": round(model / sec / 1e9 / HBM_PEAK_GBS, 4)}
-    phases = {k: {"launches": v[0], "ms_per_step": round(v[1] / args.steps, 4)} for k, v in sorted(prof.items())}
+    phases = {k: {"launches": v[0], "ms_per_step": round(v[1] / args.steps, 4)}
+              for k, v in sorted(prof["phases"].items())}
     cpu = None
     if not args.no_cpu:
         csec, nsteps, threads = run_cpu(prob)

@@ -107,6 +107,7 @@ struct Ctx {
   bool stopped = false;
   // profiling
   bool prof = false;
+  std::string prof_only;  // non-empty: time only the phase of this name
   std::map<std::string, Timer> timers;
   std::vector<void *> allocs;
 };
@@ -183,7 +184,7 @@ struct Scope {
   hipEvent_t a = nullptr, b = nullptr;
   explicit Scope(const char *n) : name(n) {
     Ctx &c = C();
-    if (!c.prof) return;
+    if (!c.prof || (!c.prof_only.empty() && c.prof_only != n)) return;
     hipEventCreate(&a);
     hipEventCreate(&b);
     hipEventRecord(a, c.st);
@@ -1684,6 +1685,10 @@ void hommexx_hip_profile(int enable) {
   Ctx &c = C();
   if (c.prof && !enable) flush_timers();
   c.prof = enable != 0;
+}
+void hommexx_hip_profile_only(const char *name) {
+  Ctx &c = C();
+  c.prof_only = name ? name : "";
 }
 void hommexx_hip_profile_reset() {
   Ctx &c = C();

@@ -116,8 +116,9 @@ __global__ __launch_bounds__(256) void euler_nbr_minmax_kernel(EulerArgs a) {
 // accumulated in FP64 the same way).
 template <typename T>
 struct LfGather {
-  const T *p[4];
-  __device__ __forceinline__ LfGather(const T *f, size_t es, int ie, int pt, int k, int s0, int s1, int s2) {
+  gptr<T> p[4];
+  __device__ __forceinline__ LfGather(const T *fg, size_t es, int ie, int pt, int k, int s0, int s1, int s2) {
+    const gptr<T> f = (gptr<T>)fg;
     p[0] = f + (size_t)ie * es + lc_index(pt, k);
     // (opaque: the selected address stays one unconditional load)
     auto at = [&](int s) { return opaque_ptr(s >= 0 ? f + (size_t)(s >> 4) * es + lc_index(s & 15, k) : p[0]); };

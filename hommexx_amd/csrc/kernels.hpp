@@ -233,11 +233,17 @@ __device__ __forceinline__ size_t opaque_index(size_t i) {
   asm volatile("" : "+v"(i));
   return i;
 }
-template <typename T>
-__device__ __forceinline__ T *opaque_ptr(T *p) {
+template <typename P>
+__device__ __forceinline__ P opaque_ptr(P p) {
   asm volatile("" : "+v"(p));
   return p;
 }
+
+// A pointer into global memory, kept in that address space: a generic
+// pointer that went through opaque_ptr would be loaded through flat
+// instructions, whose waits also cover the LDS operations.
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T *;
 
 // DSS gather of one point: the own value r[0] and up to three neighbour
 // copies r[1..3] from sources s0, s1, s2 (-1: absent), summed in that order.

@@ -53,6 +53,7 @@ def load():
         L.hommexx_hip_configure.argtypes = [_I, _I]
         L.hommexx_hip_set_abort_on_error.argtypes = [_I]
         L.hommexx_hip_profile.argtypes = [_I]
+        L.hommexx_hip_profile_only.argtypes = [ctypes.c_char_p]
         L.hommexx_hip_debug_stop.argtypes = [_I]
         L.hommexx_hip_set_form.argtypes = [_I]
         _lib = L
@@ -196,7 +197,9 @@ class HipSession:
     def synchronize(self):
         self.L.hommexx_hip_synchronize()
 
-    def profile(self, enable=True):
+    def profile(self, enable=True, only=None):
+        """HIP-event timing of the step phases; only: time just that phase"""
+        self.L.hommexx_hip_profile_only(only.encode() if only else None)
         self.L.hommexx_hip_profile(1 if enable else 0)
 
     def profile_reset(self):

@@ -138,6 +138,9 @@ void hommexx_hip_synchronize(void);
 void *hommexx_hip_stream(void);
 /* HIP-event timing of each step phase on the library stream */
 void hommexx_hip_profile(int enable);
+/* time only the phase of this name (NULL or "": every phase): fewer events in
+   the timed steps */
+void hommexx_hip_profile_only(const char *name);
 void hommexx_hip_profile_reset(void);
 int hommexx_hip_profile_get(const char *name, int *count, double *total_ms);
 int hommexx_hip_profile_names(char *buf, int len);
