@@ -391,7 +391,7 @@ void caar_run(const StateBuf &sa, const StateBuf &sb, const StateBuf &out, doubl
   x.f[2] = {c.ct, out.t, es, 16};
   x.f[3] = {c.cdp, out.dp, es, 16};
   if (lap1) {
-    x.lap1 = 1;
+    x.lap1 = c.form == 1 && c.consthv ? 2 : 1;  // the mixed form's FP32 first laplacians
     x.m = metric();
     x.lap_v = c.vt_a;
     x.lap_t = c.tt_a;
@@ -411,11 +411,11 @@ void caar_stage(int nm1, int n0, int np1, double dt, double eta_ave_w, bool lap1
   caar_run(level(nm1), level(n0), level(np1), dt, eta_ave_w, lap1);
 }
 
-// hyperviscosity's first laplacians are formed by the last RK stage's DSS:
-// FP64 hyperviscosity with at least one subcycle
+// hyperviscosity's first laplacians are formed by the last RK stage's DSS
+// (with at least one subcycle)
 bool hv_lap1_fused() {
   Ctx &c = C();
-  return c.p.hypervis_subcycle > 0 && !(c.form == 1 && c.consthv);
+  return c.p.hypervis_subcycle > 0;
 }
 
 bool stop_here(int phase) {

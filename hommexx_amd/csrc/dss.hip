@@ -13,7 +13,7 @@ namespace hx {
 // One lane = one (element, level, point) for every field of the call: the
 // source table is read once and all the fields' loads are issued before the
 // first sum, so each lane has up to 4*nf independent loads in flight.
-template <bool LAP1>
+template <int LAP1>
 __global__ __launch_bounds__(256) void dss_kernel(DssArgs a) {
   int ie, k, pt;
   if (!group_slab(a.nelem, a.nlev, ie, k, pt)) return;
@@ -75,25 +75,17 @@ __global__ __launch_bounds__(256) void dss_kernel(DssArgs a) {
   if (LAP1) {
     // the first laplacians of hyperviscosity (the work of hv_lap1_kernel on
     // the stored state), from the assembled values in registers
-    ElemLane e;
-    load_elem(a.m, ie, pt, e);
-    DvvLane d;
-    load_dvv(a.m.dvv, pt, d);
     const size_t i = ((size_t)ie * a.nlev + k) * 16 + pt, v = ((size_t)ie * a.nlev + k) * 32 + pt;
-    a.lap_t[i] = sph_laplace(e, d, y[2]);
-    a.lap_dp[i] = sph_laplace(e, d, y[3]);
-    double l0, l1;
-    sph_vlaplace_contra(e, d, a.nu_ratio1, y[0], y[1], l0, l1);
-    a.lap_v[v] = l0;
-    a.lap_v[v + 16] = l1;
+    hv_first_laplacians<LAP1 == 2>(a.m, ie, pt, a.nu_ratio1, y[2], y[3], y[0], y[1], a.lap_t, a.lap_dp, a.lap_v, i, v);
   }
 }
 
 void launch_dss(hipStream_t st, const DssArgs &a) {
   if (a.nf <= 0) return;
   const dim3 g(group_blocks_x(a.nelem, a.nlev));
-  if (a.lap1) hipLaunchKernelGGL(dss_kernel<true>, g, dim3(256), 0, st, a);
-  else hipLaunchKernelGGL(dss_kernel<false>, g, dim3(256), 0, st, a);
+  if (a.lap1 == 2) hipLaunchKernelGGL(dss_kernel<2>, g, dim3(256), 0, st, a);
+  else if (a.lap1) hipLaunchKernelGGL(dss_kernel<1>, g, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(dss_kernel<0>, g, dim3(256), 0, st, a);
 }
 
 }  // namespace hx

@@ -26,18 +26,10 @@ __device__ __forceinline__ bool slab_of(int nblk, int nelem, int nlev, int &ie, 
 __global__ __launch_bounds__(256) void hv_lap1_kernel(HvArgs a) {
   int ie, k, pt;
   if (!slab_of(gridDim.x, a.nelem, a.nlev, ie, k, pt)) return;
-  ElemLane e;
-  load_elem(a.m, ie, pt, e);
-  DvvLane d;
-  load_dvv(a.m.dvv, pt, d);
   const size_t s = ((size_t)ie * a.nlev + k) * 16 + pt;
   const size_t v = ((size_t)ie * a.nlev + k) * 32 + pt;
-  a.tt_a[s] = sph_laplace(e, d, a.t[s]);
-  a.dpt_a[s] = sph_laplace(e, d, a.dp[s]);
-  double l0, l1;
-  sph_vlaplace_contra(e, d, a.nu_ratio1, a.v[v], a.v[v + 16], l0, l1);
-  a.vt_a[v] = l0;
-  a.vt_a[v + 16] = l1;
+  hv_first_laplacians<false>(a.m, ie, pt, a.nu_ratio1, a.t[s], a.dp[s], a.v[v], a.v[v + 16], a.tt_a, a.dpt_a, a.vt_a,
+                             s, v);
 }
 
 // The first exchange is fused in: each lane gathers the assembled first
@@ -131,19 +123,10 @@ __global__ __launch_bounds__(256) void hv_lap2_pre_kernel(HvArgs a) {
 __global__ __launch_bounds__(256) void hv_lap1_mixed_kernel(HvArgs a) {
   int ie, k, pt;
   if (!slab_of(gridDim.x, a.nelem, a.nlev, ie, k, pt)) return;
-  ElemLaneF e;
-  load_elemf(a.m, ie, pt, e);
-  DvvLaneF d;
-  load_dvvf(a.m.dvv, pt, d);
   const size_t s = ((size_t)ie * a.nlev + k) * 16 + pt;
   const size_t v = ((size_t)ie * a.nlev + k) * 32 + pt;
-  const double t = a.t[s], dp = a.dp[s];
-  ((float *)a.tt_a)[s] = sph_laplacef(e, d, (float)(t - slab_ref(t)));
-  ((float *)a.dpt_a)[s] = sph_laplacef(e, d, (float)(dp - slab_ref(dp)));
-  float l0, l1;
-  sph_vlaplace_contraf(e, d, (float)a.nu_ratio1, (float)a.v[v], (float)a.v[v + 16], l0, l1);
-  ((float *)a.vt_a)[v] = l0;
-  ((float *)a.vt_a)[v + 16] = l1;
+  hv_first_laplacians<true>(a.m, ie, pt, a.nu_ratio1, a.t[s], a.dp[s], a.v[v], a.v[v + 16], a.tt_a, a.dpt_a, a.vt_a,
+                            s, v);
 }
 
 // As hv_lap2_pre: the first exchange of the FP32 first laplacian is gathered
@@ -224,11 +207,12 @@ __global__ __launch_bounds__(256) void hv_lap2_pre_mixed_kernel(HvArgs a) {
 
 // Second exchange fused with the update: the assembled tensors are used at
 // once by the lane that assembled them (same gather order as dss_kernel).
-// LAP1: the same kernel also forms the first laplacians of the updated state
-// for the next subcycle (what hv_lap1_kernel would compute from the stored
-// values), from the values in registers; they go to vt_b, tt_b, dpt_b, which
-// the caller points at the next subcycle's first-laplacian buffers.
-template <bool LAP1>
+// LAP1 (1: FP64, 2: mixed form): the same kernel also forms the first
+// laplacians of the updated state for the next subcycle (what hv_lap1_kernel
+// or hv_lap1_mixed_kernel would compute from the stored values), from the
+// values in registers; they go to vt_b, tt_b, dpt_b, which the caller points
+// at the next subcycle's first-laplacian buffers.
+template <int LAP1>
 __global__ __launch_bounds__(256) void hv_dss_update_kernel(HvArgs a) {
   int ie, k, pt;
   if (!group_slab(a.nelem, a.nlev, ie, k, pt)) return;
@@ -264,18 +248,7 @@ __global__ __launch_bounds__(256) void hv_dss_update_kernel(HvArgs a) {
   const double tn = a.t[i] + tt - heating / CP, dpn = (bdp * r);
   a.t[i] = tn;
   a.dp[i] = dpn;
-  if (LAP1) {
-    ElemLane e;
-    load_elem(a.m, ie, pt, e);
-    DvvLane d;
-    load_dvv(a.m.dvv, pt, d);
-    a.tt_b[i] = sph_laplace(e, d, tn);
-    a.dpt_b[i] = sph_laplace(e, d, dpn);
-    double l0, l1;
-    sph_vlaplace_contra(e, d, a.nu_ratio1, u, w, l0, l1);
-    a.vt_b[v0] = l0;
-    a.vt_b[v1] = l1;
-  }
+  if (LAP1) hv_first_laplacians<LAP1 == 2>(a.m, ie, pt, a.nu_ratio1, tn, dpn, u, w, a.tt_b, a.dpt_b, a.vt_b, i, v0);
 }
 
 void launch_hv_cycle(hipStream_t st, const HvArgs &a, bool lap1_done, bool lap1_next) {
@@ -285,8 +258,10 @@ void launch_hv_cycle(hipStream_t st, const HvArgs &a, bool lap1_done, bool lap1_
   // the first exchange is gathered by hv_lap2_pre, which writes the second
   // laplacian to the _b buffers; the update exchanges those (and, FP64 form,
   // may form the next subcycle's first laplacians on the way)
-  if (a.mixed) hipLaunchKernelGGL(hv_lap1_mixed_kernel, dim3(blocks), dim3(256), 0, st, a);
-  else if (!lap1_done) hipLaunchKernelGGL(hv_lap1_kernel, dim3(blocks), dim3(256), 0, st, a);
+  if (!lap1_done) {
+    if (a.mixed) hipLaunchKernelGGL(hv_lap1_mixed_kernel, dim3(blocks), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(hv_lap1_kernel, dim3(blocks), dim3(256), 0, st, a);
+  }
   if (a.mixed) hipLaunchKernelGGL(hv_lap2_pre_mixed_kernel, gg, dim3(256), 0, st, a);
   else if (a.consthv) hipLaunchKernelGGL(hv_lap2_pre_kernel<false>, gg, dim3(256), 0, st, a);
   else hipLaunchKernelGGL(hv_lap2_pre_kernel<true>, gg, dim3(256), 0, st, a);
@@ -297,8 +272,9 @@ void launch_hv_cycle(hipStream_t st, const HvArgs &a, bool lap1_done, bool lap1_
   b.vt_b = a.vt_a;
   b.tt_b = a.tt_a;
   b.dpt_b = a.dpt_a;
-  if (lap1_next && !a.mixed) hipLaunchKernelGGL(hv_dss_update_kernel<true>, gg, dim3(256), 0, st, b);
-  else hipLaunchKernelGGL(hv_dss_update_kernel<false>, gg, dim3(256), 0, st, b);
+  if (!lap1_next) hipLaunchKernelGGL(hv_dss_update_kernel<0>, gg, dim3(256), 0, st, b);
+  else if (a.mixed) hipLaunchKernelGGL(hv_dss_update_kernel<2>, gg, dim3(256), 0, st, b);
+  else hipLaunchKernelGGL(hv_dss_update_kernel<1>, gg, dim3(256), 0, st, b);
 }
 
 }  // namespace hx

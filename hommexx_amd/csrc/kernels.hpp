@@ -39,7 +39,8 @@ struct DssArgs {
   const int *src;        // [ie][16][3] global point ids (ie*16 + pt) or -1
   const double *rsmp;    // multiply by rspheremp if non-null
   // lap1 (FP64 state fields v0, v1, T, dp3d in f[0..3]): also the first
-  // hyperviscosity laplacians of the assembled state, into lap_v, lap_t, lap_dp
+  // hyperviscosity laplacians of the assembled state, into lap_v, lap_t,
+  // lap_dp (1: FP64, 2: the mixed form's FP32 laplacians)
   int lap1;
   MetricPtrs m;
   double *lap_v, *lap_t, *lap_dp;
@@ -59,6 +60,39 @@ struct HvArgs {
   int mixed;                        // FP32 laplacians, FP32 first-laplacian buffers
   int consthv;                      // 0: tensor form of the second laplacian
 };
+
+// The first laplacians of hyperviscosity at one (element, level, point) of
+// a state (t, dp, v0, v1), into the pre-exchange buffers: FP64, or (F32) the
+// mixed form's FP32 laplacians of the values less their slab reference,
+// stored as FP32.  Every lane of the slab takes part (cross-lane reads).
+template <bool F32>
+__device__ __forceinline__ void hv_first_laplacians(const MetricPtrs &m, int ie, int pt, double nu_ratio1,
+                                                    double t, double dp, double v0, double v1, double *lt,
+                                                    double *ldp, double *lv, size_t s, size_t v) {
+  if (F32) {
+    ElemLaneF e;
+    load_elemf(m, ie, pt, e);
+    DvvLaneF d;
+    load_dvvf(m.dvv, pt, d);
+    ((float *)lt)[s] = sph_laplacef(e, d, (float)(t - slab_ref(t)));
+    ((float *)ldp)[s] = sph_laplacef(e, d, (float)(dp - slab_ref(dp)));
+    float l0, l1;
+    sph_vlaplace_contraf(e, d, (float)nu_ratio1, (float)v0, (float)v1, l0, l1);
+    ((float *)lv)[v] = l0;
+    ((float *)lv)[v + 16] = l1;
+  } else {
+    ElemLane e;
+    load_elem(m, ie, pt, e);
+    DvvLane d;
+    load_dvv(m.dvv, pt, d);
+    lt[s] = sph_laplace(e, d, t);
+    ldp[s] = sph_laplace(e, d, dp);
+    double l0, l1;
+    sph_vlaplace_contra(e, d, nu_ratio1, v0, v1, l0, l1);
+    lv[v] = l0;
+    lv[v + 16] = l1;
+  }
+}
 
 // Layout of the tracer pre-exchange buffers (qdp_pre, qbih_pre, fdss_pre):
 // levels fastest in chunks of 8 levels, [ie][field][lev/8][pt][lev%8].  One
