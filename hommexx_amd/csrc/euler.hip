@@ -72,8 +72,7 @@ __global__ __launch_bounds__(256) void euler_divdp_qmm_kernel(EulerArgs a) {
     nx = a.qdp_n0[q0i + (size_t)(q + 1 < Q ? q + 1 : q) * qs];
     const double mn = slab_min(x), mx = slab_max(x);
     double *ql = a.qlim + ((size_t)ie * Q + q) * ql_row(L);
-    ql[ql_min(k)] = mn;
-    ql[ql_min(k) + 8] = mx;
+    *(double2 *)(ql + ql_min(k)) = make_double2(mn, mx);
   };
   // the first tracer is peeled so that on both paths into the loop the same
   // stores follow the prefetch (the wait then covers the prefetch only)
@@ -93,18 +92,19 @@ __global__ __launch_bounds__(256) void euler_nbr_minmax_kernel(EulerArgs a) {
   const int ie = (int)((i / L) % a.nelem);
   const int q = (int)(i / ((size_t)L * a.nelem));
   const size_t own = ((size_t)ie * Q + q) * ql_row(L) + ql_min(k);
-  double mn = a.qlim[own], mx = a.qlim[own + 8];
+  const double2 b = *(const double2 *)(a.qlim + own);
+  double mn = b.x, mx = b.y;
   // an absent neighbour (-1) reads the element's own row, which leaves the
   // min and max unchanged: no branch, all loads in flight together
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int nb = a.nbr[ie * 8 + c];
     const size_t o = opaque_index(((size_t)(nb < 0 ? ie : nb) * Q + q) * ql_row(L) + ql_min(k));
-    mx = dmax(mx, a.qlim[o + 8]);
-    mn = dmin(mn, a.qlim[o]);
+    const double2 n = *(const double2 *)(a.qlim + o);
+    mx = vmax64(mx, n.y);
+    mn = vmin64(mn, n.x);
   }
-  a.qlim_new[own] = mn;
-  a.qlim_new[own + 8] = mx;
+  *(double2 *)(a.qlim_new + own) = make_double2(mn, mx);
 }
 
 // gather of point (ie, pt) at level k from pre-exchange fields (layout
@@ -323,8 +323,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void e
   PreT *qpo_l = qpout + (size_t)ie * es + lc_index(pt, kc);
   auto fetch = [&](int q) {
     const size_t qq = q < Q ? q : Q - 1;
-    pf[2] = ql_l[qq * ql_row(L)];
-    pf[3] = ql_l[qq * ql_row(L) + 8];
+    const double2 b = *(const double2 *)(ql_l + qq * ql_row(L));
+    pf[2] = b.x;
+    pf[3] = b.y;
     pf[0] = qdp_l[qq * qs];
     pf[1] = qbp_l[qq * qs];
     if (MIXED) pf[4] = qbf_l[bihf ? qq * qs : 0];
@@ -345,7 +346,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void e
       // issued: loads complete in issue order, so waiting for them does not
       // wait for the gather
       minp = ql[0];
-      maxp = ql[8];
+      maxp = ql[QL_MAX];
     }
     if (GATHER) {
       // the stage-1 exchange of tracer q; tracer q+1's loads are issued
@@ -403,7 +404,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void e
     }
     if (pt == 0 && son) {
       if (wmin) ql[0] = minp;
-      if (wmax) ql[8] = maxp;
+      if (wmax) ql[QL_MAX] = maxp;
     }
     if (son) qpo_l[(size_t)q * fs] = (PreT)(e.sph * ptens);
   }
@@ -536,8 +537,7 @@ __global__ __launch_bounds__(256) void euler_qmm_lap_kernel(EulerArgs a) {
     const double mn = slab_min(x), mx = slab_max(x);
     if (pt == 0) {
       double *ql = a.qlim + ((size_t)ie * Q + q) * ql_row(L);
-      ql[ql_min(k)] = mn;
-      ql[ql_min(k) + 8] = mx;
+      *(double2 *)(ql + ql_min(k)) = make_double2(mn, mx);
     }
     const double y = scale ? div_r(x * wdp, rdp0) : x;
     const size_t o = ((size_t)ie * Q + q) * lc_field(L) + lc_index(pt, k);

@@ -102,10 +102,12 @@ __device__ __forceinline__ void hv_first_laplacians(const MetricPtrs &m, int ie,
 __host__ __device__ inline size_t lc_field(int nlev) { return (size_t)((nlev + 7) >> 3) * 128; }
 __host__ __device__ inline size_t lc_index(int pt, int k) { return (size_t)(k >> 3) * 128 + pt * 8 + (k & 7); }
 
-// Tracer bounds qlim: per element and tracer, levels in chunks of 8 with a
-// chunk's minima and maxima in one 128-byte line, [ie][q][lev/8][min,max][lev%8].
+// Tracer bounds qlim: per element and tracer, each level's minimum and
+// maximum side by side (one 16-byte load), [ie][q][lev][min,max], rows padded
+// to a multiple of 8 levels (one 128-byte line per 8 levels).
 __host__ __device__ inline size_t ql_row(int nlev) { return (size_t)((nlev + 7) >> 3) * 16; }
-__host__ __device__ inline size_t ql_min(int k) { return (size_t)(k >> 3) * 16 + (k & 7); }
+__host__ __device__ inline size_t ql_min(int k) { return (size_t)k * 2; }
+constexpr int QL_MAX = 1;  // offset of a level's maximum from its minimum
 
 struct EulerArgs {
   MetricPtrs m;
