@@ -87,8 +87,8 @@ struct Ctx {
   double *cv = nullptr, *ct = nullptr, *cdp = nullptr, *cdiv = nullptr;
   double *vt_a = nullptr, *tt_a = nullptr, *dpt_a = nullptr, *vt_b = nullptr, *tt_b = nullptr, *dpt_b = nullptr;
   double *qdp_pre = nullptr, *qbih = nullptr, *qbih_pre = nullptr, *qlim[2] = {}, *fdss_pre = nullptr;
-  // stage-2 pre-exchange buffers: aliases of qbih_pre and ct, which are free
-  // from the stage-2 advection until its exchange has consumed them
+  // stage-2 pre-exchange buffers: qdp_pre2 (later qbih) and an alias of ct,
+  // which is free from the stage-2 advection until its exchange has consumed it
   double *qdp_pre2 = nullptr, *fdss_pre2 = nullptr;
   int *err_d = nullptr;
   int *err_h = nullptr;
@@ -320,16 +320,17 @@ void alloc_work() {
   c.tt_b = dalloc<double>(e3);
   c.dpt_a = dalloc<double>(e3);
   c.dpt_b = dalloc<double>(e3);
-  // Three 40-F3 tracer buffers share two allocations (at Q = 40):
-  //   qdp_pre : stage-1 pre-exchange, consumed by the stage-2 gather; then
-  //             the biharmonic result qbih (bih_post -> stage-3 advection)
-  //   qbih_pre: stage-2 pre-exchange (-> euler_dss), then the first
-  //             laplacian (qmm_lap -> bih_post), then the stage-3
-  //             pre-exchange (-> euler_dss)
-  c.qdp_pre = dalloc<double>(q3c > q3 ? q3c : q3);
-  c.qbih = c.qdp_pre;
-  c.qbih_pre = dalloc<double>(q3c);
-  c.qdp_pre2 = c.qbih_pre;
+  // Five 40-F3 tracer uses share two allocations (at Q = 40); no kernel
+  // writes the allocation it gathers from:
+  //   qdp_pre : stage-1 pre-exchange (-> the stage-2 gather), then the first
+  //             laplacian of the biharmonic (the stage-2 exchange -> bih_post),
+  //             then the stage-3 pre-exchange (-> euler_dss)
+  //   qdp_pre2: stage-2 pre-exchange (-> the stage-2 exchange), then the
+  //             biharmonic result qbih (bih_post -> stage-3 advection)
+  c.qdp_pre = dalloc<double>(q3c);
+  c.qbih_pre = c.qdp_pre;
+  c.qdp_pre2 = dalloc<double>(q3c > q3 ? q3c : q3);
+  c.qbih = c.qdp_pre2;
   c.qlim[0] = dalloc<double>((size_t)c.nelem * (c.qsize ? c.qsize : 1) * ql_row(c.nlev));
   c.qlim[1] = dalloc<double>((size_t)c.nelem * (c.qsize ? c.qsize : 1) * ql_row(c.nlev));
   c.fdss_pre = dalloc<double>(e3c);
@@ -548,6 +549,7 @@ void advec_tracers(double dt) {
     a.qlim = c.qlim[cur];
   };
   nbr_minmax();
+  bool qmm_done = false;  // stage-3 bounds and first laplacian formed by the stage-2 exchange
   for (int stage = 0; stage < 3; ++stage) {
     const int n0q = stage == 0 ? c.n0_qdp : c.np1_qdp;
     a.qdp_n0 = c.qdp[n0q];
@@ -555,8 +557,8 @@ void advec_tracers(double dt) {
     a.rhs_multiplier = (double)stage;
     a.rhs_viss = stage == 2 ? 3.0 : 0.0;
     a.fdss = stage == 0 ? c.divdp_proj : stage == 1 ? c.eta_dot : c.omega;
-    // stage 3 reads qbih, which occupies the stage-1 buffer
-    a.qdp_pre = stage == 2 ? c.qbih_pre : c.qdp_pre;
+    // (stage 3 reads qbih from the other allocation)
+    a.qdp_pre = c.qdp_pre;
     a.qdp_avg_n0 = stage == 2 ? c.qdp[c.n0_qdp] : nullptr;
     // stage 2 gathers stage 1's exchange itself (the assembled stage-1 Qdp
     // is never stored) unless the caller stops after stage 1
@@ -572,8 +574,15 @@ void advec_tracers(double dt) {
         launch_euler_advect_gather(c.st, b);
       }
       b.fdss = c.eta_dot;
-      Scope s("euler_dss");
-      launch_euler_dss(c.st, b, 0);
+      // the exchange also forms the stage-3 bounds and the first laplacian
+      // of the biharmonic (the qmm_lap step below) when the form allows
+      b.rhs_multiplier = 2.0;
+      Scope s("euler_dss_qmm");
+      qmm_done = launch_euler_dss_qmm(c.st, b);
+      if (!qmm_done) {
+        b.rhs_multiplier = 1.0;
+        launch_euler_dss(c.st, b, 0);
+      }
     } else {
       {
         Scope s("euler_advect");
@@ -591,7 +600,7 @@ void advec_tracers(double dt) {
       b.qdp_n0 = c.qdp[c.np1_qdp];
       b.rhs_multiplier = 2.0;
       b.rhs_viss = 3.0;
-      {
+      if (!qmm_done) {
         Scope s("euler_qmm_lap");
         launch_euler_qmm_lap(c.st, b);
       }

@@ -452,7 +452,11 @@ __device__ __forceinline__ bool xchg_lane(int nelem, int nlev, XLane &x) {
 // of the extra field.  avg: stage 3's SSP average (Qdp(n0) + 2 Qdp(np1)) / 3
 // (:379-403) of the assembled tracer value.  F32: the tracer pre-exchange
 // values are FP32 (mixed form), summed in FP64.
-template <bool F32>
+// QMM (stage 2 only): the same block then does euler_qmm_lap_kernel's work on
+// the assembled values in LDS -- the stage-3 bounds (compute_qmin_qmax with
+// rhs_multiplier 2, a.rhs_multiplier set to 2) and the first laplacian of the
+// tracer biharmonic into qbih_pre -- so Qdp(np1) is not read back.
+template <bool F32, bool QMM>
 __global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int avg) {
   // every assembled field of the block goes into LDS before one barrier
   __shared__ double tile[QB + 1][XL][17];
@@ -489,16 +493,59 @@ __global__ __launch_bounds__(256) void euler_dss_kernel(EulerArgs a, int avg) {
     for (int i = 0; i < QB; ++i) av[i] = a.qdp_avg_n0[o0 + (size_t)(i < nq ? i : 0) * os];
   }
   lds_barrier();
-  if (wkv) {
+  if (!QMM) {
+    if (wkv) {
 #pragma unroll
-    for (int i = 0; i < QB; ++i) {
-      if (i >= nq) break;
-      double y = tile[i][x.wk][x.wpt];
-      if (avg) y = div_c<3>(av[i] + (3.0 - 1) * y);
-      st_stream(&a.qdp_np1[o0 + (size_t)i * os], y);
+      for (int i = 0; i < QB; ++i) {
+        if (i >= nq) break;
+        double y = tile[i][x.wk][x.wpt];
+        if (avg) y = div_c<3>(av[i] + (3.0 - 1) * y);
+        st_stream(&a.qdp_np1[o0 + (size_t)i * os], y);
+      }
+      if (q0 == 0) st_stream(&a.fdss[((size_t)x.ie * L + x.k0 + x.wk) * 16 + x.wpt], tile[QB][x.wk][x.wpt]);
     }
-    if (q0 == 0) st_stream(&a.fdss[((size_t)x.ie * L + x.k0 + x.wk) * 16 + x.wpt], tile[QB][x.wk][x.wpt]);
+    return;
   }
+  // stage-3 bounds and first laplacian (euler_qmm_lap_kernel's operations);
+  // every lane of a slab takes part in the cross-lane reductions
+  const int wk = x.k0 + x.wk, wkc = wkv ? wk : L - 1;
+  const size_t sl = ((size_t)x.ie * L + wkc) * 16 + x.wpt;
+  const double rhsmdt = a.rhs_multiplier * a.dt;
+  const double dp = a.derived_dp[sl] - rhsmdt * a.divdp_proj[sl];
+  const double wdp = a.dpdiss_ave[sl];
+  const Recip rdp = recip(dp), rdp0 = recip(a.dp0[wkc]);
+  const bool scale = a.nu_p > 0;
+  ElemLane e;
+  DvvLane d;
+  ElemLaneF ef;
+  DvvLaneF df;
+  if (F32) {
+    load_elemf(a.m, x.ie, x.wpt, ef);
+    load_dvvf(a.m.dvv, x.wpt, df);
+  } else {
+    load_elem_light(a.m, x.ie, x.wpt, e);
+    load_dvv(a.m.dvv, x.wpt, d);
+  }
+  double *ql = a.qlim + ((size_t)x.ie * Q + q0) * ql_row(L) + ql_min(wkc);
+  const size_t ob = ((size_t)x.ie * Q + q0) * fs + lc_index(x.wpt, wkc);
+#pragma unroll
+  for (int i = 0; i < QB; ++i) {
+    if (i >= nq) break;
+    const double y = wkv ? tile[i][x.wk][x.wpt] : 0.0;
+    if (wkv) st_stream(&a.qdp_np1[o0 + (size_t)i * os], y);
+    const double xq = div_r(y, rdp);
+    const double mn = slab_min(xq), mx = slab_max(xq);
+    if (wkv && x.wpt == 0) *(double2 *)(ql + (size_t)i * ql_row(L)) = make_double2(mn, mx);
+    const double yl = scale ? div_r(xq * wdp, rdp0) : xq;
+    if (F32) {
+      const float lap = sph_laplacef(ef, df, (float)yl);
+      if (wkv) ((float *)a.qbih_pre)[ob + (size_t)i * fs] = lap;
+    } else {
+      const double lap = sph_laplace(e, d, yl);
+      if (wkv) a.qbih_pre[ob + (size_t)i * fs] = lap;
+    }
+  }
+  if (q0 == 0 && wkv) st_stream(&a.fdss[((size_t)x.ie * L + x.k0 + x.wk) * 16 + x.wpt], tile[QB][x.wk][x.wpt]);
 }
 
 // Stage 3 compute_dp + compute_qmin_qmax (rhs_multiplier 2, :406-485) and the
@@ -642,8 +689,20 @@ void launch_euler_advect_gather(hipStream_t st, const EulerArgs &a) {
 void launch_euler_dss(hipStream_t st, const EulerArgs &a, int avg) {
   const int nq = a.qsize > 0 ? (a.qsize + QB - 1) / QB : 1;
   const dim3 g(xchg_blocks_x(a.nelem, a.nlev), nq);
-  if (a.mixed) hipLaunchKernelGGL(euler_dss_kernel<true>, g, dim3(16 * XL), 0, st, a, avg);
-  else hipLaunchKernelGGL(euler_dss_kernel<false>, g, dim3(16 * XL), 0, st, a, avg);
+  if (a.mixed) hipLaunchKernelGGL((euler_dss_kernel<true, false>), g, dim3(16 * XL), 0, st, a, avg);
+  else hipLaunchKernelGGL((euler_dss_kernel<false, false>), g, dim3(16 * XL), 0, st, a, avg);
+}
+
+bool launch_euler_dss_qmm(hipStream_t st, const EulerArgs &a) {
+  // Used in the mixed form (FP32 pre-exchange values and FP32 biharmonic
+  // buffers), where it measured 1.72 ms against 1.90 for the two launches.
+  // In FP64 the fused kernel (88 VGPRs, 5 waves per SIMD against 7) measured
+  // 2.13 ms against 2.08, so FP64 keeps two launches, as does the mixed form
+  // with tensor hyperviscosity (FP64 biharmonic buffers).
+  if (a.qsize <= 0 || !a.mixed || !a.bih32) return false;
+  const dim3 g(xchg_blocks_x(a.nelem, a.nlev), (a.qsize + QB - 1) / QB);
+  hipLaunchKernelGGL((euler_dss_kernel<true, true>), g, dim3(16 * XL), 0, st, a, 0);
+  return true;
 }
 
 void launch_euler_qmm_lap(hipStream_t st, const EulerArgs &a) {

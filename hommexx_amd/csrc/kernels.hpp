@@ -220,6 +220,9 @@ void launch_euler_nbr_minmax(hipStream_t st, const EulerArgs &a);
 void launch_euler_advect(hipStream_t st, const EulerArgs &a, int stage);
 void launch_euler_advect_gather(hipStream_t st, const EulerArgs &a);
 void launch_euler_dss(hipStream_t st, const EulerArgs &a, int avg);
+// the stage-2 exchange with the stage-3 bounds and first laplacian of the
+// biharmonic (a.rhs_multiplier 2); false: not launched (form not covered)
+bool launch_euler_dss_qmm(hipStream_t st, const EulerArgs &a);
 void launch_euler_qmm_lap(hipStream_t st, const EulerArgs &a);
 void launch_euler_bih_post(hipStream_t st, const EulerArgs &a);
 void launch_state_forcing(hipStream_t st, int nelem, int nlev, double dt, const double *ft, const double *fm,
