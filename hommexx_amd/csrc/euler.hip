@@ -331,7 +331,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void e
     if (MIXED) pf[4] = qbf_l[bihf ? qq * qs : 0];
   };
   if (PF && Q > 0) fetch(0);
-  for (int q = 0; q < Q; ++q) {
+  // tracer q's work; at stage 2 gcur holds tracer q's gathered loads and
+  // takes tracer q+GD's.  The mixed form (FP32 gathers, 96 VGPRs) keeps two
+  // tracers of gathers in flight (1581 against 1664 us); FP64 one (two need
+  // 112 VGPRs, 4 waves per SIMD: 1820 against 1708 us).
+  constexpr int GD = MIXED ? 2 : 1;
+  auto tracer = [&](int q, PreT (&gcur)[4]) {
     const size_t qi = (size_t)q * qs;  // tracer q's offset from qdp_l, qbp_l, qbf_l
     double *ql = ql_l + (size_t)q * ql_row(L);
     double minp, maxp, qd, q0pf = 0.0;
@@ -349,14 +354,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void e
       maxp = ql[QL_MAX];
     }
     if (GATHER) {
-      // the stage-1 exchange of tracer q; tracer q+1's loads are issued
+      // the stage-1 exchange of tracer q; tracer q+GD's loads are issued
       // before this tracer's arithmetic and stay in flight across the
       // (LDS-only) barrier
-      const double y = gather_sum(graw, g0, g1, g2) * gr;
+      const double y = gather_sum(gcur, g0, g1, g2) * gr;
       if (gon) tile[q & 1][gkk][gpt] = y;
       // (unconditional, the last tracer reloads itself: a branch here would
       // make every later wait cover the gathers)
-      gq.load((size_t)(q + 1 < Q ? q + 1 : q) * fs, graw);
+      gq.load((size_t)(q + GD < Q ? q + GD : Q - 1) * fs, gcur);
       lds_barrier();
       qd = tile[q & 1][kk][pt];
     } else if (!PF) {
@@ -407,6 +412,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void e
       if (wmax) ql[QL_MAX] = maxp;
     }
     if (son) qpo_l[(size_t)q * fs] = (PreT)(e.sph * ptens);
+  };
+  if (GATHER && GD == 2) {
+    PreT gnext[4] = {};
+    if (Q > 1) gq.load(fs, gnext);
+    int q = 0;
+    for (; q + 1 < Q; q += 2) {
+      tracer(q, graw);
+      tracer(q + 1, gnext);
+    }
+    if (q < Q) tracer(q, graw);
+  } else {
+    for (int q = 0; q < Q; ++q) tracer(q, graw);
   }
 }
 
