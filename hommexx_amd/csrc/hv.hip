@@ -39,10 +39,14 @@ __global__ __launch_bounds__(256) void hv_lap1_kernel(HvArgs a) {
 // inputs of a neighbour's gather are never overwritten during the launch).
 // Slabs are walked in level groups (group_slab) so neighbour copies come
 // from L2.
-template <bool TENSOR>
+// TOP: the nu_top sponge levels 0..2 only (a launch of their own, so that the
+// sponge's extra laplacians do not raise the register count of the main
+// launch: 123 VGPRs without them, 152 with); !TOP with nu_top > 0 skips them.
+template <bool TENSOR, bool TOP>
 __global__ __launch_bounds__(256) void hv_lap2_pre_kernel(HvArgs a) {
   int ie, k, pt;
-  if (!group_slab(a.nelem, a.nlev, ie, k, pt)) return;
+  if (!group_slab(a.nelem, TOP ? (a.nlev < 3 ? a.nlev : 3) : a.nlev, ie, k, pt)) return;
+  if (!TOP && a.nu_top > 0 && k < 3) return;  // whole slab
   const size_t s = ((size_t)ie * a.nlev + k) * 16 + pt;
   const size_t v = ((size_t)ie * a.nlev + k) * 32 + pt;
   double g[4];  // assembled vt (2 components), tt, dpt
@@ -79,7 +83,7 @@ __global__ __launch_bounds__(256) void hv_lap2_pre_kernel(HvArgs a) {
   // and stored before the next one starts (fewer live registers)
   double tv[4], vs[6];
   if (TENSOR) load_tensor(a.m, ie, pt, tv, vs);
-  const bool top = a.nu_top > 0 && k < 3;  // uniform over the slab
+  const bool top = TOP;  // (the TOP launch runs only with nu_top > 0)
   const double sc = (k == 0 ? 4.0 : k == 1 ? 2.0 : 1.0) * a.nu_top;
   {
     double vt0, vt1;
@@ -263,8 +267,13 @@ void launch_hv_cycle(hipStream_t st, const HvArgs &a, bool lap1_done, bool lap1_
     else hipLaunchKernelGGL(hv_lap1_kernel, dim3(blocks), dim3(256), 0, st, a);
   }
   if (a.mixed) hipLaunchKernelGGL(hv_lap2_pre_mixed_kernel, gg, dim3(256), 0, st, a);
-  else if (a.consthv) hipLaunchKernelGGL(hv_lap2_pre_kernel<false>, gg, dim3(256), 0, st, a);
-  else hipLaunchKernelGGL(hv_lap2_pre_kernel<true>, gg, dim3(256), 0, st, a);
+  else if (a.consthv) hipLaunchKernelGGL((hv_lap2_pre_kernel<false, false>), gg, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL((hv_lap2_pre_kernel<true, false>), gg, dim3(256), 0, st, a);
+  if (!a.mixed && a.nu_top > 0) {
+    const dim3 gt(group_blocks_x(a.nelem, a.nlev < 3 ? a.nlev : 3));
+    if (a.consthv) hipLaunchKernelGGL((hv_lap2_pre_kernel<false, true>), gt, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((hv_lap2_pre_kernel<true, true>), gt, dim3(256), 0, st, a);
+  }
   HvArgs b = a;
   b.vt_a = a.vt_b;
   b.tt_a = a.tt_b;
