@@ -103,3 +103,44 @@ def test_build_dimensions_without_configure():
         assert e <= 1e-12, (k, e)
     assert np.array_equal(out["Qdp"][:, :, 3:], prob.state["Qdp"][:, :, 3:])
     assert not np.any(out["Q"][:, 3:])
+
+
+def test_phase_timers():
+    """hommexx_hip_profile_only restricts the HIP-event timers to one phase
+    (the timed steps of bench.py carry only those events); without it every
+    phase of the step is timed.  The timers do not change the results."""
+    prob = build_problem(ne=2, nlev=26, qsize=4, seed=45)
+    with HipSession(prob) as h:
+        h.profile(True, only="euler_advect")
+        h.run(2)
+        h.profile(False)
+        one = h.profile_report()
+        h.profile_reset()
+        h.profile(True)
+        h.run(1)
+        h.profile(False)
+        every = h.profile_report()
+        timed = h.state()
+    assert set(one) == {"euler_advect"}
+    assert one["euler_advect"][0] == 2 * 3 and one["euler_advect"][1] > 0  # three stages per step
+    assert {"caar", "caar_dss", "hv_cycle", "euler_advect", "euler_dss", "remap"} <= set(every)
+    with HipSession(prob) as h:
+        h.run(3)
+        plain = h.state()
+    for k in ("v", "T", "dp3d", "Qdp"):
+        assert np.array_equal(timed[k], plain[k]), k
+
+
+def test_run_steps_driver(capsys):
+    """python -m hommexx_amd.run_steps, at a small size, in both forms"""
+    import sys
+    from hommexx_amd import run_steps
+    for form in ("fp64", "mixed"):
+        argv = sys.argv
+        sys.argv = ["run_steps", "--form", form, "--ne", "2", "--nlev", "26", "--qsize", "4", "--steps", "2",
+                    "--warmup", "1"]
+        try:
+            run_steps.main()
+        finally:
+            sys.argv = argv
+        assert ("%s ne2 L26 Q4: 2 steps" % form) in capsys.readouterr().out

@@ -77,6 +77,8 @@ def test_ten_step_parity():
     dict(nu_div=2.5e16),
     dict(moisture=True),
     dict(qsplit=2, hypervis_subcycle=2),
+    dict(nu_top=0.0),  # no sponge launch: the second-laplacian kernel covers every level
+    dict(hypervis_subcycle=1),  # the first laplacians come only from the last RK stage's DSS
 ])
 def test_option_parity(over):
     prob = build_problem(ne=4, nlev=26, qsize=3, seed=3, **over)

@@ -29,7 +29,9 @@ def errors(a, b):
     return {k: rel_l2(a[k], b[k]) for k in PROGNOSTIC}
 
 
-@pytest.mark.parametrize("ne,nlev,qsize,nsteps", [(4, 26, 4, 10), (8, 72, 4, 10)])
+# (qsize 5: an odd tracer count, for the stage-2 gather that keeps two
+# tracers of gathers in flight and finishes with a single one)
+@pytest.mark.parametrize("ne,nlev,qsize,nsteps", [(4, 26, 4, 10), (8, 72, 4, 10), (4, 26, 5, 10)])
 def test_mixed_form_within_1e6_of_fp64(ne, nlev, qsize, nsteps):
     prob = build_problem(ne=ne, nlev=nlev, qsize=qsize, seed=21, t_noise=0.1)
     ref, tl = run_form(prob, "fp64", nsteps)
