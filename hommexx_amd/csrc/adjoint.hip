@@ -484,53 +484,84 @@ struct RCol {
 };
 
 // grid factors of PpmRemap.hpp:367-413 at j from X = dx(j..j+3), with their
-// partial derivatives dG[m] = dG/dX[m]
+// partial derivatives dG[m] = dG/dX[m].  One division per factor: the value
+// is N/Dn correctly rounded (div_r from the reciprocal r = 1/Dn, the same
+// value as the forward remap's division); the derivatives are products with r
+// (N/Dn^2 as (N/Dn)*r: within an ulp or two of the quotients, which the
+// gradient's tolerance does not see).
 __device__ __forceinline__ double ppm_g(int i, const double X[4], double dG[4]) {
   const double X0 = X[0], X1 = X[1], X2 = X[2], X3 = X[3];
   dG[0] = dG[1] = dG[2] = dG[3] = 0.0;
-  double N, Dn, S;
+  double N, Dn;
   switch (i) {
-    case 0:
-      S = X0 + X1 + X2;
-      dG[0] = -X1 / (S * S); dG[1] = (X0 + X2) / (S * S); dG[2] = -X1 / (S * S);
-      return X1 / S;
-    case 1:
+    case 0: {
+      Dn = X0 + X1 + X2;
+      const Recip r = recip(Dn);
+      const double r2 = r.y * r.y;
+      dG[0] = -X1 * r2; dG[1] = (X0 + X2) * r2; dG[2] = -X1 * r2;
+      return div_r(X1, r);
+    }
+    case 1: {
       N = 2.0 * X0 + X1; Dn = X1 + X2;
-      dG[0] = 2.0 / Dn; dG[1] = 1.0 / Dn - N / (Dn * Dn); dG[2] = -N / (Dn * Dn);
-      return N / Dn;
-    case 2:
+      const Recip r = recip(Dn);
+      const double q = div_r(N, r);
+      dG[0] = 2.0 * r.y; dG[1] = r.y - q * r.y; dG[2] = -q * r.y;
+      return q;
+    }
+    case 2: {
       N = X1 + 2.0 * X2; Dn = X0 + X1;
-      dG[0] = -N / (Dn * Dn); dG[1] = 1.0 / Dn - N / (Dn * Dn); dG[2] = 2.0 / Dn;
-      return N / Dn;
-    case 3:
+      const Recip r = recip(Dn);
+      const double q = div_r(N, r);
+      dG[0] = -q * r.y; dG[1] = r.y - q * r.y; dG[2] = 2.0 * r.y;
+      return q;
+    }
+    case 3: {
       Dn = X1 + X2;
-      dG[1] = X2 / (Dn * Dn); dG[2] = -X1 / (Dn * Dn);
-      return X1 / Dn;
+      const Recip r = recip(Dn);
+      const double r2 = r.y * r.y;
+      dG[1] = X2 * r2; dG[2] = -X1 * r2;
+      return div_r(X1, r);
+    }
     case 4: {
       const double g4 = 1.0 / (X0 + X1 + X2 + X3);
       dG[0] = dG[1] = dG[2] = dG[3] = -g4 * g4;
       return g4;
     }
-    case 5:
+    case 5: {
       Dn = X1 + X2;
-      dG[1] = 2.0 * X2 * X2 / (Dn * Dn); dG[2] = 2.0 * X1 * X1 / (Dn * Dn);
-      return 2.0 * X1 * X2 / Dn;
-    case 6:
+      const Recip r = recip(Dn);
+      const double r2 = r.y * r.y;
+      dG[1] = 2.0 * X2 * X2 * r2; dG[2] = 2.0 * X1 * X1 * r2;
+      return div_r(2.0 * X1 * X2, r);
+    }
+    case 6: {
       N = X0 + X1; Dn = 2.0 * X1 + X2;
-      dG[0] = 1.0 / Dn; dG[1] = 1.0 / Dn - 2.0 * N / (Dn * Dn); dG[2] = -N / (Dn * Dn);
-      return N / Dn;
-    case 7:
+      const Recip r = recip(Dn);
+      const double q = div_r(N, r);
+      dG[0] = r.y; dG[1] = r.y - 2.0 * q * r.y; dG[2] = -q * r.y;
+      return q;
+    }
+    case 7: {
       N = X3 + X2; Dn = 2.0 * X2 + X1;
-      dG[1] = -N / (Dn * Dn); dG[2] = 1.0 / Dn - 2.0 * N / (Dn * Dn); dG[3] = 1.0 / Dn;
-      return N / Dn;
-    case 8:
+      const Recip r = recip(Dn);
+      const double q = div_r(N, r);
+      dG[1] = -q * r.y; dG[2] = r.y - 2.0 * q * r.y; dG[3] = r.y;
+      return q;
+    }
+    case 8: {
       N = X1 * (X0 + X1); Dn = 2.0 * X1 + X2;
-      dG[0] = X1 / Dn; dG[1] = (X0 + 2.0 * X1) / Dn - 2.0 * N / (Dn * Dn); dG[2] = -N / (Dn * Dn);
-      return N / Dn;
-    default:
+      const Recip r = recip(Dn);
+      const double q = div_r(N, r);
+      dG[0] = X1 * r.y; dG[1] = (X0 + 2.0 * X1) * r.y - 2.0 * q * r.y; dG[2] = -q * r.y;
+      return q;
+    }
+    default: {
       N = X2 * (X2 + X3); Dn = X1 + 2.0 * X2;
-      dG[1] = -N / (Dn * Dn); dG[2] = (2.0 * X2 + X3) / Dn - 2.0 * N / (Dn * Dn); dG[3] = X2 / Dn;
-      return N / Dn;
+      const Recip r = recip(Dn);
+      const double q = div_r(N, r);
+      dG[1] = -q * r.y; dG[2] = (2.0 * X2 + X3) * r.y - 2.0 * q * r.y; dG[3] = X2 * r.y;
+      return q;
+    }
   }
 }
 
