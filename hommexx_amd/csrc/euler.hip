@@ -62,22 +62,35 @@ __global__ __launch_bounds__(256) void euler_divdp_qmm_kernel(EulerArgs a) {
   const double rhsmdt = a.rhs_multiplier * a.dt;
   const double dp = a.derived_dp[s] - rhsmdt * dv;
   const Recip rdp = recip(dp);  // Qdp / dp for every tracer
-  // tracer q+1's Qdp is loaded before tracer q's reductions; every lane
-  // stores the slab's bounds (the same value: measured faster here than a
-  // store from one lane, whose branch makes the next wait cover the stores)
+  // the Qdp of the next PD tracers is in flight during a tracer's
+  // reductions: slot j of nx holds tracers j, j+PD, j+2PD, ..., refilled as
+  // it is consumed (the loop body covers PD tracers, so no value moves
+  // between registers: a move would wait for its load).  Every lane stores
+  // the slab's bounds (the same value: measured faster here than a store
+  // from one lane, whose branch makes the next wait cover the stores).
+  if (Q <= 0) return;
+  constexpr int PD = 4;
   const size_t q0i = ((size_t)ie * Q * L + k) * 16 + pt, qs = (size_t)L * 16;
-  double nx = Q > 0 ? a.qdp_n0[q0i] : 0.0;
-  auto tracer = [&](int q) {
-    const double x = div_r(nx, rdp);
-    nx = a.qdp_n0[q0i + (size_t)(q + 1 < Q ? q + 1 : q) * qs];
+  // (past the last tracer the last is reloaded: the loads stay unconditional)
+  auto ld = [&](int q) { return a.qdp_n0[q0i + (size_t)(q < Q ? q : Q - 1) * qs]; };
+  double nx[PD];
+#pragma unroll
+  for (int j = 0; j < PD; ++j) nx[j] = ld(j);
+  auto tracer = [&](int q, double &slot) {
+    const double x = div_r(slot, rdp);
+    slot = ld(q + PD);
     const double mn = slab_min(x), mx = slab_max(x);
     double *ql = a.qlim + ((size_t)ie * Q + q) * ql_row(L);
     *(double2 *)(ql + ql_min(k)) = make_double2(mn, mx);
   };
-  // the first tracer is peeled so that on both paths into the loop the same
-  // stores follow the prefetch (the wait then covers the prefetch only)
-  if (Q > 0) tracer(0);
-  for (int q = 1; q < Q; ++q) tracer(q);
+  int q = 0;
+  for (; q + PD <= Q; q += PD) {
+#pragma unroll
+    for (int j = 0; j < PD; ++j) tracer(q + j, nx[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < PD; ++j)
+    if (q + j < Q) tracer(q + j, nx[j]);
 }
 
 // min/max of qlim over the element and its (up to 8) neighbours (:487-507).
